@@ -1,0 +1,239 @@
+/*
+ * m1lya.h -- C ABI of libm1lya, the MI355X (gfx950) multi-frequency M1 Lya
+ * radiative-transfer step of LYDION.
+ *
+ * The library is the drop-in behind the Julia operator API that
+ * src/StaticCloudTest.jl and src/RHDSolver_*.jl call (olofnebrin/LYDION):
+ * every entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status: M1LYA_OK (0) or a negative code.
+ *    m1lya_last_error() returns the message of the last failure.  No C++
+ *    exception crosses this boundary.
+ *  - Host arrays use the reference (Julia column-major) layout, so a Julia
+ *    Array{Float64,3} of size (N_ν, N_r, N_Z) is passed as-is:
+ *      spectral fields  [N_ν, N_r, N_Z]   (k fastest)
+ *      cell fields      [N_r, N_Z]
+ *      u_r_star, A_r    [N_r+1, N_Z]
+ *      u_Z_star, A_Z    [N_r, N_Z+1]
+ *  - Bin indices (kcore_min/kmax, active-bin ranges) are 0-based here; the
+ *    Julia shim subtracts 1 from LyαArrays.kcore_min/kcore_max.
+ *  - A context is not re-entrant: one host thread at a time.  All work is
+ *    queued on the context's HIP stream; functions that return host data
+ *    synchronise that stream.  Host pointers are never retained.
+ *  - State stays resident in device memory between calls; upload/download
+ *    it at save or checkpoint points.
+ */
+#ifndef M1LYA_H
+#define M1LYA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M1LYA_ABI_VERSION 1
+
+typedef struct m1lya_ctx m1lya_ctx;
+
+enum m1lya_status {
+    M1LYA_OK = 0,
+    M1LYA_ERR_ARG = -1,        /* bad argument (NULL pointer, wrong count, bad enum) */
+    M1LYA_ERR_DIMS = -2,       /* unsupported dimensions (N_r < 3, N_Z < 3, N_ν < 3) */
+    M1LYA_ERR_HIP = -3,        /* HIP runtime error (allocation, launch)            */
+    M1LYA_ERR_NONFINITE = -4,  /* non-finite value detected in the state             */
+    M1LYA_ERR_STATE = -5,      /* geometry / frequency grid not set                  */
+    M1LYA_ERR_UNSUPPORTED = -6 /* option not implemented                             */
+};
+
+/* Symbol BCs of the reference (:Reflect, :Outflow). */
+enum m1lya_bc { M1LYA_REFLECT = 0, M1LYA_OUTFLOW = 1 };
+
+/* LyαArrays{FluxMethod}: Val{:GLF} or Val{:HLL} (src/SetupLymanAlphaArrays.jl:220-224). */
+enum m1lya_flux { M1LYA_GLF = 0, M1LYA_HLL = 1 };
+
+/* Arithmetic of the step. */
+enum m1lya_precision { M1LYA_FP64 = 0 };
+
+/* Device-resident fields, named after the LyαArrays / solver arrays they mirror. */
+enum m1lya_field {
+    /* [N_ν, N_r, N_Z] */
+    M1LYA_J = 0,          /* J_Lyα        */
+    M1LYA_J_OLD,          /* J_Lyα_old    */
+    M1LYA_H_R,            /* H_r_Lyα      */
+    M1LYA_H_Z,            /* H_Z_Lyα      */
+    M1LYA_H_R_OLD,        /* H_r_Lyα_old  */
+    M1LYA_H_Z_OLD,        /* H_Z_Lyα_old  */
+    M1LYA_ALPHA_F,        /* α_F_Lyα      */
+    M1LYA_HVOIGT,         /* Hvoigt       */
+    M1LYA_PHI_NU,         /* ϕ_ν          */
+    M1LYA_FIJ_KP,         /* Fij_kp       */
+    M1LYA_FIJ_K,          /* Fij_k        */
+    M1LYA_FIJ_KM,         /* Fij_km       */
+    /* [N_r, N_Z] */
+    M1LYA_ALPHA_LYA,      /* α_Lyα        */
+    M1LYA_ALPHA_C,        /* αc_Lyα       */
+    M1LYA_PD,             /* pd           */
+    M1LYA_JS,             /* ∫dν_js_Lyα   */
+    M1LYA_T,              /* gas temperature T                       */
+    M1LYA_DNU_D,          /* ΔνD          */
+    M1LYA_DUR_DR,         /* ∂u_r/∂r      */
+    M1LYA_DUR_DZ,         /* ∂u_r/∂Z      */
+    M1LYA_DUZ_DR,         /* ∂u_Z/∂r      */
+    M1LYA_DUZ_DZ,         /* ∂u_Z/∂Z      */
+    M1LYA_UR_OVER_R,      /* ⟨u_r/r⟩      */
+    M1LYA_DRHOU_R,        /* Δρu_Lyα_r    */
+    M1LYA_DRHOU_Z,        /* Δρu_Lyα_Z    */
+    M1LYA_F_LYA_R,        /* f_Lyα_r      */
+    M1LYA_F_LYA_Z,        /* f_Lyα_Z      */
+    M1LYA_E_LYA,          /* e_Lyα        */
+    /* [N_r+1, N_Z] */
+    M1LYA_UR_STAR,        /* u_r_star     */
+    /* [N_r, N_Z+1] */
+    M1LYA_UZ_STAR,        /* u_Z_star     */
+    M1LYA_NFIELDS
+};
+
+/* GeometryCylindrical{Float64} (src/Structs.jl:650-740), the fields the M1
+ * step reads.  Interval arrays have N entries (N = N_r or N_Z) and follow
+ * Compute_Grid_Differences! (src/SpatialGrid.jl:516-586); the *_inner and
+ * *_outer arrays are the {ΔL, ΔR, inv(i,i-1), inv(i+1,i), inv(i+2,i+1)}
+ * values of the first and last interface. */
+typedef struct {
+    const double *r, *Z;            /* [N_r+1], [N_Z+1]  */
+    const double *rCoM, *ZCoM;      /* [N_r], [N_Z]      */
+    const double *dV, *inv_dV;      /* [N_r, N_Z]        */
+    const double *A_r;              /* [N_r+1, N_Z]      */
+    const double *A_Z;              /* [N_r, N_Z+1]      */
+    const double *inv_r_cell;       /* [N_r]             */
+    const double *dr_L, *dr_R, *inv_dr_i_im1, *inv_dr_ip1_i, *inv_dr_ip2_ip1;
+    double dr_inner[5], dr_outer[5];
+    const double *dZ_L, *dZ_R, *inv_dZ_j_jm1, *inv_dZ_jp1_j, *inv_dZ_jp2_jp1;
+    double dZ_inner[5], dZ_outer[5];
+} m1lya_geometry;
+
+/* Frequency grid and settings of LyαArrays (src/Structs.jl:23-124). */
+typedef struct {
+    const double *nu;          /* ν, [N_ν+1] interfaces  */
+    const double *nuC;         /* νC, [N_ν]              */
+    const double *dnu;         /* Δν, [N_ν]              */
+    const double *inv_dnu;     /* inv_Δν, [N_ν]          */
+    const double *inv_nuC;     /* inv_νC, [N_ν]          */
+    int kcore_min, kcore_max;  /* 0-based, inclusive     */
+    int recoil;                /* Recoil                 */
+    int flux_method;           /* enum m1lya_flux        */
+    double alpha_floor;        /* α_Lyα_floor            */
+} m1lya_freq_grid;
+
+/* ---- context --------------------------------------------------------- */
+int  m1lya_abi_version(void);
+/* Creates a context for an N_ν x N_r x N_Z cylindrical problem on the current
+ * HIP device (GenerateLyαArrays, src/SetupLymanAlphaArrays.jl:17-239, state
+ * allocation; all state starts at zero like the reference's zeros()). */
+int  m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx **out);
+void m1lya_destroy(m1lya_ctx *ctx);
+/* Copies the last error message (ctx may be NULL: last create() failure). */
+int  m1lya_last_error(const m1lya_ctx *ctx, char *buf, size_t len);
+int  m1lya_set_geometry(m1lya_ctx *ctx, const m1lya_geometry *geometry);
+int  m1lya_set_freq_grid(m1lya_ctx *ctx, const m1lya_freq_grid *grid);
+/* count must equal the field's element count. */
+int  m1lya_upload(m1lya_ctx *ctx, int field, const double *host, size_t count);
+int  m1lya_download(m1lya_ctx *ctx, int field, double *host, size_t count);
+int  m1lya_fill(m1lya_ctx *ctx, int field, double value);
+int  m1lya_synchronize(m1lya_ctx *ctx);
+
+/* ---- operator entry points (drop-ins for the reference functions) ----- */
+/* Velocity / temperature arrays may be NULL: the device-resident field is used. */
+
+/* FindActiveBins, src/UpdateLyaJandH.jl:47-97 (0-based kmin/kmax). */
+int m1lya_find_active_bins(m1lya_ctx *ctx, int *kmin, int *kmax);
+
+/* AdvectLyα!(c̃Δt, Lyα, geometry, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r,
+ * ∂u_Z_∂Z, u_r_over_r, BC_r_left, BC_r_right, BC_Z_bot, BC_Z_top),
+ * src/UpdateLyaJandH.jl:107-171 -> AdvectJandH_Cylindrical!, src/AdvectJandH.jl:526-1390.
+ * BC_r_left must be M1LYA_REFLECT (cylindrical axis). */
+int m1lya_advect_lya(m1lya_ctx *ctx, double c_dt,
+                     const double *u_r_star, const double *u_z_star,
+                     const double *dur_dr, const double *dur_dz,
+                     const double *duz_dr, const double *duz_dz, const double *ur_over_r,
+                     int bc_r_left, int bc_r_right, int bc_z_bot, int bc_z_top);
+
+/* LyαEmissionAbsorptionUpdate!(J, J_old, H_r, H_Z, H_r_old, H_Z_old, ∫dν_js, ϕ_ν,
+ * Hvoigt, α_Lyα, αc, α_F, pd, Δρu_r, Δρu_Z, Δν, c̃, Δt),
+ * src/ConstantsAndFunctions.jl:483-573.  All arrays are the device-resident fields. */
+int m1lya_emission_absorption_update(m1lya_ctx *ctx, double c_tilde, double dt);
+
+/* UpdateFreqFactors!(Lyα, T, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r),
+ * src/FreqFactors.jl:142-278: fills the Fij_kp / Fij_k / Fij_km fields. */
+int m1lya_update_freq_factors(m1lya_ctx *ctx, const double *T,
+                              const double *dur_dr, const double *dur_dz,
+                              const double *duz_dr, const double *duz_dz, const double *ur_over_r);
+
+/* FrequencyUpdateJLyα_AllCells!(c̃Δt, Lyα, T, ∂u..., u_r_over_r, UpdateFreqFactors),
+ * src/UpdateLyaJandH.jl:231-391 (TridiagonalSolve -> LAPACK gtsv, src/TridiagonalSolver.jl:33-58). */
+int m1lya_frequency_update_all_cells(m1lya_ctx *ctx, double c_dt, const double *T,
+                                     const double *dur_dr, const double *dur_dz,
+                                     const double *duz_dr, const double *duz_dz,
+                                     const double *ur_over_r, int update_freq_factors);
+
+/* Update_f_Lyα_MF_L_Lyα!(f_Lyα_r, f_Lyα_Z, Δρu_r, Δρu_Z, ∫dν_js, ΔV, rCoM, ZCoM, Δt,
+ * Z_center, BC_Z_bot), src/ConstantsAndFunctions.jl:588-674.
+ * out = {M_F, M_F_radial_net, M_F_radial_out, L_Lyα_tot}; fills f_Lyα_r/Z. */
+int m1lya_update_f_mf_l(m1lya_ctx *ctx, double dt, double z_center, int bc_z_bot, double out[4]);
+
+/* Gravity variant, src/ConstantsAndFunctions.jl:691-804 (f_grav arrays [N_r, N_Z]).
+ * out = {M_F, M_F_antigravity, M_F_radial_net, M_F_radial_out, L_Lyα_tot}. */
+int m1lya_update_f_mf_l_gravity(m1lya_ctx *ctx, double dt, const double *f_grav_r,
+                                const double *f_grav_z, double z_center, int bc_z_bot,
+                                double out[5]);
+
+/* Compute_f_esc_Lyα_Cylindrical(H_r, H_Z, A_r, A_Z, Δν, L_Lyα_allspace, BC_Z_bot),
+ * src/ConstantsAndFunctions.jl:969-993. */
+int m1lya_compute_f_esc(m1lya_ctx *ctx, double L_allspace, int bc_z_bot, double *f_esc);
+
+/* Reset_Δρu!, src/ConstantsAndFunctions.jl:809-816. */
+int m1lya_reset_drhou(m1lya_ctx *ctx);
+
+/* Update_e_Lyα!, src/ConstantsAndFunctions.jl:416-446 (fills M1LYA_E_LYA). */
+int m1lya_update_e_lya(m1lya_ctx *ctx);
+
+/* Edge spectra written by SaveTimeStepHDF5, src/SaveHDF5Data_StaticCloudTests.jl:152-200,
+ * 392-420.  raw/norm are [N_edge, N_ν] row-major (edge slowest), N_edge = m1lya_edge_count(). */
+int m1lya_edge_count(const m1lya_ctx *ctx);
+int m1lya_edge_spectra(m1lya_ctx *ctx, double *raw, double *norm, size_t count);
+
+/* ---- fused step ------------------------------------------------------ */
+/* One pass of the RT-only time loop body, src/StaticCloudTest.jl:357-392:
+ * AdvectLyα! -> LyαEmissionAbsorptionUpdate! -> FrequencyUpdateJLyα_AllCells! ->
+ * Update_f_Lyα_MF_L_Lyα! -> Compute_f_esc_Lyα_Cylindrical -> Reset_Δρu!,
+ * fused into device kernels with no host round trip.  The velocity, T and
+ * coefficient fields are the device-resident ones.  BC_r_left is the axis
+ * (always reflecting).  out (may be NULL) = {M_F, M_F_radial_net,
+ * M_F_radial_out, L_Lyα, f_esc, kmin, kmax, 0}; out != NULL synchronises. */
+int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
+               int bc_r_right, int bc_z_bot, int bc_z_top, double out[8]);
+
+/* n_steps fused steps without host synchronisation.  history (may be NULL)
+ * receives n_steps rows of the 8 step scalars. */
+int m1lya_run_steps(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
+                    int bc_r_right, int bc_z_bot, int bc_z_top, double *history);
+
+/* ---- instrumentation ------------------------------------------------- */
+/* When on, every kernel launch is bracketed by HIP events on the context's
+ * stream; m1lya_kernel_stats() reports per-kernel launch counts and total
+ * device milliseconds (synchronises). */
+int m1lya_set_profiling(m1lya_ctx *ctx, int on);
+int m1lya_kernel_stats_count(m1lya_ctx *ctx);
+int m1lya_kernel_stats(m1lya_ctx *ctx, int index, char *name, size_t name_len,
+                       long *launches, double *total_ms);
+int m1lya_reset_kernel_stats(m1lya_ctx *ctx);
+
+/* Runs n IEEE operations on the device for the bit-exactness self-test:
+ * op 0: a/b, 1: sqrt(a), 2: the library's exp(a). */
+int m1lya_selftest_math(int op, const double *a, const double *b, double *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,928 @@
+// m1lya_api.hip -- the C ABI of libm1lya (include/m1lya.h): context, device
+// memory, the operator entry points that replace the reference Julia
+// functions, and the fused time step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/m1lya.h"
+#include "m1lya_internal.h"
+
+using namespace m1;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct KStat {
+    long launches = 0;
+    double total_ms = 0.0;
+};
+
+struct Pending {
+    std::string name;
+    hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct m1lya_ctx {
+    int nn = 0, nr = 0, nz = 0, precision = 0;
+    size_t ncell = 0, n3 = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // spectral state: cur set and the second set ("old" arrays / ping-pong target)
+    double *J = nullptr, *Hr = nullptr, *Hz = nullptr;
+    double *Jo = nullptr, *Hro = nullptr, *Hzo = nullptr;
+    bool old_stale = false;   // after fused steps the *_old arrays equal the current state
+    bool fij_stale = false;   // Fij_* not maintained by fused steps
+    bool krange_valid = false;
+
+    double *aF = nullptr, *Hv = nullptr, *phi = nullptr;
+    double *Fkp = nullptr, *Fk = nullptr, *Fkm = nullptr;
+
+    double *alpha = nullptr, *alpha_c = nullptr, *pd = nullptr, *js = nullptr, *T = nullptr, *dnuD = nullptr;
+    double *dur_dr = nullptr, *dur_dz = nullptr, *duz_dr = nullptr, *duz_dz = nullptr, *ur_over_r = nullptr;
+    double *drhou_r = nullptr, *drhou_z = nullptr, *f_r = nullptr, *f_z = nullptr, *e_lya = nullptr;
+    double *ur_star = nullptr, *uz_star = nullptr;
+
+    // geometry
+    double *A_r = nullptr, *A_z = nullptr, *dV = nullptr, *inv_dV = nullptr, *inv_r_cell = nullptr;
+    double *rcom = nullptr, *zcom = nullptr, *ivr = nullptr, *ivz = nullptr;
+    bool geom_set = false;
+
+    // frequency grid
+    double *nu = nullptr, *nuC = nullptr, *dnu = nullptr, *inv_dnu = nullptr, *inv_nuC = nullptr;
+    int kcore_min = 0, kcore_max = 0, recoil = 0, flux = 0;
+    double alpha_floor = 0.0;
+    bool grid_set = false;
+
+    // reductions and scratch
+    int col_blocks = 0, act_blocks = 0, pstride = 0, prows = 0;
+    double *part = nullptr, *red = nullptr, *scalars = nullptr;
+    int *krange = nullptr, *err_flag = nullptr;
+    double *edge_buf = nullptr;
+    double *fg_r = nullptr, *fg_z = nullptr;
+    double *history = nullptr;
+    size_t history_cap = 0;
+
+    std::vector<void*> allocs;
+    std::string err;
+
+    bool profiling = false;
+    std::map<std::string, KStat> stats;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+struct Fail {
+    int code;
+};
+
+void fail(m1lya_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    g_last_error = msg;
+    throw Fail{code};
+}
+
+void hipchk(m1lya_ctx* c, hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(c, M1LYA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double* dalloc(m1lya_ctx* c, size_t n)
+{
+    void* p = nullptr;
+    hipchk(c, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)), "hipMalloc");
+    c->allocs.push_back(p);
+    hipchk(c, hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(double)), "hipMemset");
+    return (double*)p;
+}
+
+int* ialloc(m1lya_ctx* c, size_t n)
+{
+    void* p = nullptr;
+    hipchk(c, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(int)), "hipMalloc");
+    c->allocs.push_back(p);
+    hipchk(c, hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(int)), "hipMemset");
+    return (int*)p;
+}
+
+hipEvent_t get_event(m1lya_ctx* c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipchk(c, hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// Brackets one kernel launch with events on the context stream.
+struct Prof {
+    m1lya_ctx* c;
+    const char* name;
+    hipEvent_t e0{}, e1{};
+    Prof(m1lya_ctx* c_, const char* n) : c(c_), name(n)
+    {
+        if (c->profiling) {
+            e0 = get_event(c);
+            e1 = get_event(c);
+            hipchk(c, hipEventRecord(e0, c->stream), "hipEventRecord");
+        }
+    }
+    void done()
+    {
+        hipchk(c, hipGetLastError(), name);
+        if (c->profiling) {
+            hipchk(c, hipEventRecord(e1, c->stream), "hipEventRecord");
+            c->pending.push_back(Pending{name, e0, e1});
+        }
+    }
+};
+
+void collect_stats(m1lya_ctx* c)
+{
+    if (c->pending.empty()) return;
+    hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        hipchk(c, hipEventElapsedTime(&ms, p.e0, p.e1), "hipEventElapsedTime");
+        KStat& s = c->stats[p.name];
+        s.launches += 1;
+        s.total_ms += ms;
+        c->pool.push_back(p.e0);
+        c->pool.push_back(p.e1);
+    }
+    c->pending.clear();
+}
+
+struct FieldInfo {
+    double* ptr;
+    size_t count;
+};
+
+FieldInfo field_info(m1lya_ctx* c, int f, bool for_download)
+{
+    const size_t n3 = c->n3, nc = c->ncell;
+    const bool stale = c->old_stale && for_download;
+    switch (f) {
+    case M1LYA_J: return {c->J, n3};
+    case M1LYA_J_OLD: return {stale ? c->J : c->Jo, n3};
+    case M1LYA_H_R: return {c->Hr, n3};
+    case M1LYA_H_Z: return {c->Hz, n3};
+    case M1LYA_H_R_OLD: return {stale ? c->Hr : c->Hro, n3};
+    case M1LYA_H_Z_OLD: return {stale ? c->Hz : c->Hzo, n3};
+    case M1LYA_ALPHA_F: return {c->aF, n3};
+    case M1LYA_HVOIGT: return {c->Hv, n3};
+    case M1LYA_PHI_NU: return {c->phi, n3};
+    case M1LYA_FIJ_KP: return {c->Fkp, n3};
+    case M1LYA_FIJ_K: return {c->Fk, n3};
+    case M1LYA_FIJ_KM: return {c->Fkm, n3};
+    case M1LYA_ALPHA_LYA: return {c->alpha, nc};
+    case M1LYA_ALPHA_C: return {c->alpha_c, nc};
+    case M1LYA_PD: return {c->pd, nc};
+    case M1LYA_JS: return {c->js, nc};
+    case M1LYA_T: return {c->T, nc};
+    case M1LYA_DNU_D: return {c->dnuD, nc};
+    case M1LYA_DUR_DR: return {c->dur_dr, nc};
+    case M1LYA_DUR_DZ: return {c->dur_dz, nc};
+    case M1LYA_DUZ_DR: return {c->duz_dr, nc};
+    case M1LYA_DUZ_DZ: return {c->duz_dz, nc};
+    case M1LYA_UR_OVER_R: return {c->ur_over_r, nc};
+    case M1LYA_DRHOU_R: return {c->drhou_r, nc};
+    case M1LYA_DRHOU_Z: return {c->drhou_z, nc};
+    case M1LYA_F_LYA_R: return {c->f_r, nc};
+    case M1LYA_F_LYA_Z: return {c->f_z, nc};
+    case M1LYA_E_LYA: return {c->e_lya, nc};
+    case M1LYA_UR_STAR: return {c->ur_star, (size_t)(c->nr + 1) * c->nz};
+    case M1LYA_UZ_STAR: return {c->uz_star, (size_t)c->nr * (c->nz + 1)};
+    default: fail(c, M1LYA_ERR_ARG, "unknown field id " + std::to_string(f));
+    }
+    return {nullptr, 0};
+}
+
+// After fused steps the *_old arrays hold a stale state; the reference
+// invariant at step boundaries is J_old == J, H_old == H.  Materialise it
+// before an operator-level call reads or writes the old arrays.
+void materialize_old(m1lya_ctx* c)
+{
+    if (!c->old_stale) return;
+    size_t b = c->n3 * sizeof(double);
+    hipchk(c, hipMemcpyAsync(c->Jo, c->J, b, hipMemcpyDeviceToDevice, c->stream), "copy J_old");
+    hipchk(c, hipMemcpyAsync(c->Hro, c->Hr, b, hipMemcpyDeviceToDevice, c->stream), "copy H_r_old");
+    hipchk(c, hipMemcpyAsync(c->Hzo, c->Hz, b, hipMemcpyDeviceToDevice, c->stream), "copy H_Z_old");
+    c->old_stale = false;
+}
+
+void require_setup(m1lya_ctx* c)
+{
+    if (!c->geom_set) fail(c, M1LYA_ERR_STATE, "geometry not set (m1lya_set_geometry)");
+    if (!c->grid_set) fail(c, M1LYA_ERR_STATE, "frequency grid not set (m1lya_set_freq_grid)");
+}
+
+void check_bc(m1lya_ctx* c, int bc, const char* which)
+{
+    if (bc != M1LYA_REFLECT && bc != M1LYA_OUTFLOW)
+        fail(c, M1LYA_ERR_ARG, std::string("invalid boundary condition for ") + which);
+}
+
+void upload_opt(m1lya_ctx* c, double* dst, const double* src, size_t n)
+{
+    if (src) hipchk(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
+}
+
+void upload_velocity(m1lya_ctx* c, const double* dur_dr, const double* dur_dz, const double* duz_dr,
+                     const double* duz_dz, const double* ur_over_r)
+{
+    upload_opt(c, c->dur_dr, dur_dr, c->ncell);
+    upload_opt(c, c->dur_dz, dur_dz, c->ncell);
+    upload_opt(c, c->duz_dr, duz_dr, c->ncell);
+    upload_opt(c, c->duz_dz, duz_dz, c->ncell);
+    upload_opt(c, c->ur_over_r, ur_over_r, c->ncell);
+}
+
+void compute_krange(m1lya_ctx* c)
+{
+    Prof p1(c, "active_partial");
+    launch_active_partial(c->stream, c->J, c->dnu, c->dV, c->nn, (int)c->ncell, c->act_blocks, c->part,
+                          c->pstride);
+    p1.done();
+    Prof p2(c, "reduce_cols");
+    launch_reduce_cols(c->stream, c->part, c->act_blocks, c->pstride, 2 * c->nn, c->red);
+    p2.done();
+    Prof p3(c, "active_final");
+    launch_active_final(c->stream, c->red, c->nn, c->kcore_min, c->kcore_max, c->krange);
+    p3.done();
+    c->krange_valid = true;
+}
+
+AdvectArgs advect_args(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top)
+{
+    AdvectArgs a;
+    a.nn = c->nn;
+    a.g = DevGeom{c->nr, c->nz, c->A_r, c->A_z, c->inv_dV, c->inv_r_cell, c->ivr, c->ivz};
+    a.v = DevVel{c->ur_star, c->uz_star, c->dur_dr, c->dur_dz, c->duz_dr, c->duz_dz, c->ur_over_r};
+    a.J = c->J; a.Hr = c->Hr; a.Hz = c->Hz; a.aF = c->aF;
+    a.J_out = c->Jo; a.Hr_out = c->Hro; a.Hz_out = c->Hzo;
+    a.cdt = cdt;
+    a.bc_r_right = bc_r_right; a.bc_z_bot = bc_z_bot; a.bc_z_top = bc_z_top;
+    a.hll = c->flux;
+    a.krange = c->krange;
+    return a;
+}
+
+// Advection into the second buffer set, then swap roles.
+void run_advect(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top)
+{
+    AdvectArgs a = advect_args(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    Prof p(c, "advect");
+    launch_advect(c->stream, a);
+    p.done();
+    std::swap(c->J, c->Jo);
+    std::swap(c->Hr, c->Hro);
+    std::swap(c->Hz, c->Hzo);
+}
+
+ColumnArgs column_args(m1lya_ctx* c, int flags)
+{
+    ColumnArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nn = c->nn; a.nr = c->nr; a.nz = c->nz; a.flags = flags;
+    a.J = c->J; a.Hr = c->Hr; a.Hz = c->Hz;
+    a.J_old = c->Jo; a.Hr_old = c->Hro; a.Hz_old = c->Hzo;
+    a.Hv = c->Hv; a.phi = c->phi; a.aF = c->aF;
+    a.Fkp = c->Fkp; a.Fk = c->Fk; a.Fkm = c->Fkm;
+    a.alpha = c->alpha; a.alpha_c = c->alpha_c; a.pd = c->pd; a.js = c->js; a.T = c->T; a.dnuD = c->dnuD;
+    a.dur_dr = c->dur_dr; a.dur_dz = c->dur_dz; a.duz_dr = c->duz_dr; a.duz_dz = c->duz_dz;
+    a.ur_over_r = c->ur_over_r;
+    a.drhou_r = c->drhou_r; a.drhou_z = c->drhou_z; a.f_r = c->f_r; a.f_z = c->f_z;
+    a.nu = c->nu; a.nuC = c->nuC; a.dnu = c->dnu; a.inv_dnu = c->inv_dnu; a.inv_nuC = c->inv_nuC;
+    a.dV = c->dV; a.rcom = c->rcom; a.zcom = c->zcom; a.A_r = c->A_r; a.A_z = c->A_z;
+    a.alpha_floor = c->alpha_floor; a.recoil = c->recoil;
+    a.c_tilde = 1.0; a.dt = 1.0; a.dw = 0.0;
+    a.part = c->part; a.pstride = c->pstride; a.err_flag = c->err_flag;
+    return a;
+}
+
+void check_err_flag(m1lya_ctx* c, const char* what)
+{
+    int h = 0;
+    hipchk(c, hipMemcpyAsync(&h, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream), "flag");
+    hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    if (h) {
+        hipchk(c, hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream), "flag reset");
+        fail(c, M1LYA_ERR_NONFINITE, std::string(what) + ": singular tridiagonal system (gtsv INFO > 0)");
+    }
+}
+
+void fused_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc_r_right, int bc_z_bot,
+                int bc_z_top, double* history_row)
+{
+    if (!c->krange_valid) compute_krange(c);
+    const double cdt = c_tilde * dt;
+    run_advect(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED);
+    a.c_tilde = c_tilde; a.dt = dt; a.dw = 1.0 / cdt; a.z_center = z_center; a.bc_z_bot = bc_z_bot;
+    Prof p(c, "column");
+    launch_column(c->stream, a, c->col_blocks);
+    p.done();
+    Prof p2(c, "reduce_cols");
+    launch_reduce_cols(c->stream, c->part, c->col_blocks, c->pstride, 2 * c->nn + 5, c->red);
+    p2.done();
+    Prof p3(c, "step_final");
+    launch_step_final(c->stream, c->red, c->nn, c->kcore_min, c->kcore_max, bc_z_bot, c->krange, c->scalars,
+                      history_row);
+    p3.done();
+    c->old_stale = true;
+    c->fij_stale = true;
+    c->krange_valid = true;
+}
+
+template <class F>
+int guarded(m1lya_ctx* c, F&& f)
+{
+    if (!c) {
+        g_last_error = "null context";
+        return M1LYA_ERR_ARG;
+    }
+    try {
+        f();
+        return M1LYA_OK;
+    } catch (const Fail& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        g_last_error = e.what();
+        return M1LYA_ERR_HIP;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int m1lya_abi_version(void) { return M1LYA_ABI_VERSION; }
+
+int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
+{
+    if (!out) {
+        g_last_error = "out is NULL";
+        return M1LYA_ERR_ARG;
+    }
+    *out = nullptr;
+    if (n_nu < 3 || n_r < 3 || n_z < 3) {
+        g_last_error = "need N_ν >= 3, N_r >= 3, N_Z >= 3 (src/SpatialGrid.jl:629-630)";
+        return M1LYA_ERR_DIMS;
+    }
+    if (n_nu > 256) {
+        g_last_error = "N_ν > 256 is not supported by the column kernel";
+        return M1LYA_ERR_DIMS;
+    }
+    if (precision != M1LYA_FP64) {
+        g_last_error = "unsupported precision";
+        return M1LYA_ERR_UNSUPPORTED;
+    }
+    m1lya_ctx* c = new m1lya_ctx();
+    int rc = guarded(c, [&] {
+        c->nn = n_nu; c->nr = n_r; c->nz = n_z; c->precision = precision;
+        c->ncell = (size_t)n_r * n_z;
+        c->n3 = c->ncell * n_nu;
+        hipchk(c, hipGetDevice(&c->device), "hipGetDevice");
+        hipchk(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        const size_t n3 = c->n3, nc = c->ncell;
+        c->J = dalloc(c, n3); c->Hr = dalloc(c, n3); c->Hz = dalloc(c, n3);
+        c->Jo = dalloc(c, n3); c->Hro = dalloc(c, n3); c->Hzo = dalloc(c, n3);
+        c->aF = dalloc(c, n3); c->Hv = dalloc(c, n3); c->phi = dalloc(c, n3);
+        c->Fkp = dalloc(c, n3); c->Fk = dalloc(c, n3); c->Fkm = dalloc(c, n3);
+        double** cells[] = {&c->alpha, &c->alpha_c, &c->pd, &c->js, &c->T, &c->dnuD, &c->dur_dr, &c->dur_dz,
+                            &c->duz_dr, &c->duz_dz, &c->ur_over_r, &c->drhou_r, &c->drhou_z, &c->f_r,
+                            &c->f_z, &c->e_lya, &c->dV, &c->inv_dV, &c->fg_r, &c->fg_z};
+        for (double** p : cells) *p = dalloc(c, nc);
+        c->ur_star = dalloc(c, (size_t)(n_r + 1) * n_z);
+        c->uz_star = dalloc(c, (size_t)n_r * (n_z + 1));
+        c->A_r = dalloc(c, (size_t)(n_r + 1) * n_z);
+        c->A_z = dalloc(c, (size_t)n_r * (n_z + 1));
+        c->inv_r_cell = dalloc(c, n_r); c->rcom = dalloc(c, n_r); c->zcom = dalloc(c, n_z);
+        c->ivr = dalloc(c, (size_t)(n_r + 1) * 5); c->ivz = dalloc(c, (size_t)(n_z + 1) * 5);
+        c->nu = dalloc(c, n_nu + 1); c->nuC = dalloc(c, n_nu); c->dnu = dalloc(c, n_nu);
+        c->inv_dnu = dalloc(c, n_nu); c->inv_nuC = dalloc(c, n_nu);
+        const int W = column_width();
+        const int ngroups = (int)((nc + W - 1) / W);
+        c->col_blocks = std::min(ngroups, 2560);
+        c->act_blocks = (int)std::min<size_t>(nc, 1024);
+        c->pstride = 2 * n_nu + 8;
+        c->prows = std::max(c->col_blocks, c->act_blocks);
+        c->part = dalloc(c, (size_t)c->prows * c->pstride);
+        c->red = dalloc(c, c->pstride);
+        c->scalars = dalloc(c, 8);
+        c->krange = ialloc(c, 2);
+        c->err_flag = ialloc(c, 1);
+        int ne = (n_z - 2) + 2 * (n_r - 3);
+        c->edge_buf = dalloc(c, (size_t)std::max(ne, 1) * n_nu * 2);
+        hipchk(c, hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+    if (rc != M1LYA_OK) {
+        m1lya_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return M1LYA_OK;
+}
+
+void m1lya_destroy(m1lya_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->history) (void)hipFree(c->history);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int m1lya_last_error(const m1lya_ctx* c, char* buf, size_t len)
+{
+    if (!buf || len == 0) return M1LYA_ERR_ARG;
+    const std::string& s = c ? c->err : g_last_error;
+    std::snprintf(buf, len, "%s", s.c_str());
+    return M1LYA_OK;
+}
+
+int m1lya_set_geometry(m1lya_ctx* c, const m1lya_geometry* g)
+{
+    return guarded(c, [&] {
+        if (!g || !g->dV || !g->inv_dV || !g->A_r || !g->A_Z || !g->inv_r_cell || !g->rCoM || !g->ZCoM ||
+            !g->dr_L || !g->dr_R || !g->inv_dr_i_im1 || !g->inv_dr_ip1_i || !g->inv_dr_ip2_ip1 || !g->dZ_L ||
+            !g->dZ_R || !g->inv_dZ_j_jm1 || !g->inv_dZ_jp1_j || !g->inv_dZ_jp2_jp1)
+            fail(c, M1LYA_ERR_ARG, "geometry has a NULL array");
+        const int nr = c->nr, nz = c->nz;
+        auto up = [&](double* d, const double* h, size_t n) {
+            hipchk(c, hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice), "geometry upload");
+        };
+        up(c->dV, g->dV, c->ncell);
+        up(c->inv_dV, g->inv_dV, c->ncell);
+        up(c->A_r, g->A_r, (size_t)(nr + 1) * nz);
+        up(c->A_z, g->A_Z, (size_t)nr * (nz + 1));
+        up(c->inv_r_cell, g->inv_r_cell, nr);
+        up(c->rcom, g->rCoM, nr);
+        up(c->zcom, g->ZCoM, nz);
+        // One interval set per interface: face 0 = inner boundary, face N = outer
+        // boundary, face f in 1..N-1 uses the interior arrays at f-1
+        // (src/AdvectJandH.jl:632-639, 803-812, 1092-1098, 1105-1114).
+        auto pack = [&](int n, const double* dL, const double* dR, const double* a, const double* b,
+                        const double* cc, const double* inner, const double* outer, double* dst) {
+            std::vector<double> h((size_t)(n + 1) * 5);
+            for (int m = 0; m < 5; ++m) { h[m] = inner[m]; h[(size_t)n * 5 + m] = outer[m]; }
+            for (int f = 1; f < n; ++f) {
+                h[(size_t)f * 5 + 0] = dL[f - 1];
+                h[(size_t)f * 5 + 1] = dR[f - 1];
+                h[(size_t)f * 5 + 2] = a[f - 1];
+                h[(size_t)f * 5 + 3] = b[f - 1];
+                h[(size_t)f * 5 + 4] = cc[f - 1];
+            }
+            up(dst, h.data(), h.size());
+        };
+        pack(nr, g->dr_L, g->dr_R, g->inv_dr_i_im1, g->inv_dr_ip1_i, g->inv_dr_ip2_ip1, g->dr_inner, g->dr_outer,
+             c->ivr);
+        pack(nz, g->dZ_L, g->dZ_R, g->inv_dZ_j_jm1, g->inv_dZ_jp1_j, g->inv_dZ_jp2_jp1, g->dZ_inner, g->dZ_outer,
+             c->ivz);
+        c->geom_set = true;
+        c->krange_valid = false;
+    });
+}
+
+int m1lya_set_freq_grid(m1lya_ctx* c, const m1lya_freq_grid* g)
+{
+    return guarded(c, [&] {
+        if (!g || !g->nu || !g->nuC || !g->dnu || !g->inv_dnu || !g->inv_nuC)
+            fail(c, M1LYA_ERR_ARG, "frequency grid has a NULL array");
+        if (g->kcore_min < 0 || g->kcore_max >= c->nn || g->kcore_min > g->kcore_max)
+            fail(c, M1LYA_ERR_ARG, "kcore range out of bounds");
+        if (g->flux_method != M1LYA_GLF && g->flux_method != M1LYA_HLL)
+            fail(c, M1LYA_ERR_ARG, "unknown flux method");
+        const int nn = c->nn;
+        auto up = [&](double* d, const double* h, size_t n) {
+            hipchk(c, hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice), "grid upload");
+        };
+        up(c->nu, g->nu, nn + 1);
+        up(c->nuC, g->nuC, nn);
+        up(c->dnu, g->dnu, nn);
+        up(c->inv_dnu, g->inv_dnu, nn);
+        up(c->inv_nuC, g->inv_nuC, nn);
+        c->kcore_min = g->kcore_min; c->kcore_max = g->kcore_max;
+        c->recoil = g->recoil ? 1 : 0;
+        c->flux = g->flux_method;
+        c->alpha_floor = g->alpha_floor;
+        c->grid_set = true;
+        c->krange_valid = false;
+    });
+}
+
+int m1lya_upload(m1lya_ctx* c, int field, const double* host, size_t count)
+{
+    return guarded(c, [&] {
+        if (!host) fail(c, M1LYA_ERR_ARG, "host pointer is NULL");
+        FieldInfo fi = field_info(c, field, false);
+        if (count != fi.count)
+            fail(c, M1LYA_ERR_ARG, "field " + std::to_string(field) + ": count " + std::to_string(count) +
+                                       " != " + std::to_string(fi.count));
+        if (field <= M1LYA_H_Z_OLD) materialize_old(c);
+        if (field >= M1LYA_FIJ_KP && field <= M1LYA_FIJ_KM) c->fij_stale = false;
+        hipchk(c, hipMemcpyAsync(fi.ptr, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream),
+               "upload");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        if (field == M1LYA_J) c->krange_valid = false;
+    });
+}
+
+int m1lya_download(m1lya_ctx* c, int field, double* host, size_t count)
+{
+    return guarded(c, [&] {
+        if (!host) fail(c, M1LYA_ERR_ARG, "host pointer is NULL");
+        if (field >= M1LYA_FIJ_KP && field <= M1LYA_FIJ_KM && c->fij_stale)
+            fail(c, M1LYA_ERR_STATE, "Fij_* are not maintained by the fused step; call "
+                                     "m1lya_update_freq_factors first");
+        FieldInfo fi = field_info(c, field, true);
+        if (count != fi.count)
+            fail(c, M1LYA_ERR_ARG, "field " + std::to_string(field) + ": count " + std::to_string(count) +
+                                       " != " + std::to_string(fi.count));
+        hipchk(c, hipMemcpyAsync(host, fi.ptr, count * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+               "download");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_fill(m1lya_ctx* c, int field, double value)
+{
+    return guarded(c, [&] {
+        FieldInfo fi = field_info(c, field, false);
+        if (field <= M1LYA_H_Z_OLD) materialize_old(c);
+        std::vector<double> h(fi.count, value);
+        hipchk(c, hipMemcpyAsync(fi.ptr, h.data(), fi.count * sizeof(double), hipMemcpyHostToDevice, c->stream),
+               "fill");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        if (field == M1LYA_J) c->krange_valid = false;
+    });
+}
+
+int m1lya_synchronize(m1lya_ctx* c)
+{
+    return guarded(c, [&] {
+        hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        check_err_flag(c, "step");
+    });
+}
+
+int m1lya_find_active_bins(m1lya_ctx* c, int* kmin, int* kmax)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        compute_krange(c);
+        int h[2];
+        hipchk(c, hipMemcpyAsync(h, c->krange, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "krange");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        if (kmin) *kmin = h[0];
+        if (kmax) *kmax = h[1];
+    });
+}
+
+int m1lya_advect_lya(m1lya_ctx* c, double c_dt, const double* u_r_star, const double* u_z_star,
+                     const double* dur_dr, const double* dur_dz, const double* duz_dr, const double* duz_dz,
+                     const double* ur_over_r, int bc_r_left, int bc_r_right, int bc_z_bot, int bc_z_top)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (bc_r_left != M1LYA_REFLECT)
+            fail(c, M1LYA_ERR_ARG, "cylindrical geometry: the axis boundary must be M1LYA_REFLECT");
+        check_bc(c, bc_r_right, "BC_r_right");
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        check_bc(c, bc_z_top, "BC_Z_top");
+        materialize_old(c);
+        upload_opt(c, c->ur_star, u_r_star, (size_t)(c->nr + 1) * c->nz);
+        upload_opt(c, c->uz_star, u_z_star, (size_t)c->nr * (c->nz + 1));
+        upload_velocity(c, dur_dr, dur_dz, duz_dr, duz_dz, ur_over_r);
+        compute_krange(c);   // FindActiveBins (src/UpdateLyaJandH.jl:153)
+        run_advect(c, c_dt, bc_r_right, bc_z_bot, bc_z_top);
+        // J_old = J (src/AdvectJandH.jl:1385-1387); H_old = H (src/UpdateLyaJandH.jl:167-168)
+        size_t b = c->n3 * sizeof(double);
+        hipchk(c, hipMemcpyAsync(c->Jo, c->J, b, hipMemcpyDeviceToDevice, c->stream), "J_old");
+        hipchk(c, hipMemcpyAsync(c->Hro, c->Hr, b, hipMemcpyDeviceToDevice, c->stream), "H_r_old");
+        hipchk(c, hipMemcpyAsync(c->Hzo, c->Hz, b, hipMemcpyDeviceToDevice, c->stream), "H_Z_old");
+        c->krange_valid = false;
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_emission_absorption_update(m1lya_ctx* c, double c_tilde, double dt)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        materialize_old(c);
+        ColumnArgs a = column_args(c, COL_EMIT);
+        a.c_tilde = c_tilde; a.dt = dt;
+        Prof p(c, "column_emit");
+        launch_column(c->stream, a, c->col_blocks);
+        p.done();
+        c->krange_valid = false;
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_update_freq_factors(m1lya_ctx* c, const double* T, const double* dur_dr, const double* dur_dz,
+                              const double* duz_dr, const double* duz_dz, const double* ur_over_r)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        upload_opt(c, c->T, T, c->ncell);
+        upload_velocity(c, dur_dr, dur_dz, duz_dr, duz_dz, ur_over_r);
+        ColumnArgs a = column_args(c, COL_FREQ | COL_STORE_F);
+        Prof p(c, "column_freq_factors");
+        launch_column(c->stream, a, c->col_blocks);
+        p.done();
+        c->fij_stale = false;
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_frequency_update_all_cells(m1lya_ctx* c, double c_dt, const double* T, const double* dur_dr,
+                                     const double* dur_dz, const double* duz_dr, const double* duz_dz,
+                                     const double* ur_over_r, int update_freq_factors)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (!update_freq_factors && c->fij_stale)
+            fail(c, M1LYA_ERR_STATE, "Fij_* are stale (fused steps); call with update_freq_factors = 1");
+        materialize_old(c);
+        upload_opt(c, c->T, T, c->ncell);
+        upload_velocity(c, dur_dr, dur_dz, duz_dr, duz_dz, ur_over_r);
+        int flags = COL_SOLVE | COL_HCORR | (update_freq_factors ? (COL_FREQ | COL_STORE_F) : 0);
+        ColumnArgs a = column_args(c, flags);
+        a.dw = 1.0 / c_dt;
+        Prof p(c, "column_freq_update");
+        launch_column(c->stream, a, c->col_blocks);
+        p.done();
+        if (update_freq_factors) c->fij_stale = false;
+        c->krange_valid = false;
+        check_err_flag(c, "FrequencyUpdateJLyα_AllCells");
+    });
+}
+
+static void mf_common(m1lya_ctx* c, double dt, const double* fgr, const double* fgz, double z_center,
+                      double* sums)
+{
+    MfArgs a;
+    a.nr = c->nr; a.nz = c->nz;
+    a.drhou_r = c->drhou_r; a.drhou_z = c->drhou_z; a.f_r = c->f_r; a.f_z = c->f_z;
+    a.js = c->js; a.dV = c->dV; a.rcom = c->rcom; a.zcom = c->zcom;
+    a.fg_r = fgr; a.fg_z = fgz; a.dt = dt; a.z_center = z_center; a.part = c->part;
+    int nb = (int)std::min<size_t>((c->ncell + 255) / 256, 512);
+    Prof p(c, "mf_partial");
+    launch_mf_partial(c->stream, a, nb);
+    p.done();
+    launch_reduce_cols(c->stream, c->part, nb, 8, 5, c->red);
+    hipchk(c, hipGetLastError(), "reduce");
+    hipchk(c, hipMemcpyAsync(sums, c->red, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "sums");
+    hipchk(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int m1lya_update_f_mf_l(m1lya_ctx* c, double dt, double z_center, int bc_z_bot, double out[4])
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        if (!out) fail(c, M1LYA_ERR_ARG, "out is NULL");
+        double s[5];
+        mf_common(c, dt, nullptr, nullptr, z_center, s);
+        // normalisation: src/ConstantsAndFunctions.jl:663-673
+        double MF = s[0], L = s[1], net = s[2], ow = s[3];
+        if (L > 0.0) {
+            double f = 1.0 / (L / kC);
+            MF = MF * f; net = net * f; ow = ow * f;
+        } else {
+            MF = 0.0; net = 0.0; ow = 0.0;
+        }
+        out[0] = MF; out[1] = net; out[2] = ow;
+        out[3] = (bc_z_bot == M1LYA_REFLECT) ? 2.0 * L : L;
+    });
+}
+
+int m1lya_update_f_mf_l_gravity(m1lya_ctx* c, double dt, const double* f_grav_r, const double* f_grav_z,
+                                double z_center, int bc_z_bot, double out[5])
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        if (!out || !f_grav_r || !f_grav_z) fail(c, M1LYA_ERR_ARG, "NULL argument");
+        upload_opt(c, c->fg_r, f_grav_r, c->ncell);
+        upload_opt(c, c->fg_z, f_grav_z, c->ncell);
+        double s[5];
+        mf_common(c, dt, c->fg_r, c->fg_z, z_center, s);
+        double MF = s[0], L = s[1], net = s[2], ow = s[3], ag = s[4];
+        if (L > 0.0) {
+            double f = 1.0 / (L / kC);
+            MF = MF * f; ag = ag * f; net = net * f; ow = ow * f;
+        } else {
+            MF = 0.0; ag = 0.0; net = 0.0; ow = 0.0;
+        }
+        out[0] = MF; out[1] = ag; out[2] = net; out[3] = ow;
+        out[4] = (bc_z_bot == M1LYA_REFLECT) ? 2.0 * L : L;
+    });
+}
+
+int m1lya_compute_f_esc(m1lya_ctx* c, double L_allspace, int bc_z_bot, double* f_esc)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        if (!f_esc) fail(c, M1LYA_ERR_ARG, "f_esc is NULL");
+        if (L_allspace <= 0.0) {
+            *f_esc = 0.0;
+            return;
+        }
+        int nb = 256;
+        Prof p(c, "fesc_partial");
+        launch_fesc_partial(c->stream, c->Hr, c->Hz, c->A_r, c->A_z, c->dnu, c->nn, c->nr, c->nz, bc_z_bot, nb,
+                            c->part);
+        p.done();
+        launch_reduce_cols(c->stream, c->part, nb, 1, 1, c->red);
+        double s = 0.0;
+        hipchk(c, hipMemcpyAsync(&s, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream), "sum");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        double Lsim = (bc_z_bot == M1LYA_REFLECT) ? 0.5 * L_allspace : L_allspace;
+        *f_esc = (Lsim > 0.0) ? s / Lsim : 0.0;
+    });
+}
+
+int m1lya_reset_drhou(m1lya_ctx* c)
+{
+    return guarded(c, [&] {
+        hipchk(c, hipMemsetAsync(c->drhou_r, 0, c->ncell * sizeof(double), c->stream), "memset");
+        hipchk(c, hipMemsetAsync(c->drhou_z, 0, c->ncell * sizeof(double), c->stream), "memset");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_update_e_lya(m1lya_ctx* c)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        Prof p(c, "e_lya");
+        launch_e_lya(c->stream, c->J, c->dnu, c->nn, (int)c->ncell, c->e_lya);
+        p.done();
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_edge_count(const m1lya_ctx* c)
+{
+    if (!c) return 0;
+    return (c->nz - 2) + 2 * (c->nr - 3);
+}
+
+int m1lya_edge_spectra(m1lya_ctx* c, double* raw, double* norm, size_t count)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        size_t ne = (size_t)m1lya_edge_count(c);
+        if (!raw || !norm || count != ne * c->nn) fail(c, M1LYA_ERR_ARG, "edge spectra: bad buffers/count");
+        double* draw = c->edge_buf;
+        double* dnorm = c->edge_buf + ne * c->nn;
+        launch_edge(c->stream, c->J, c->dnu, c->nn, c->nr, c->nz, draw, dnorm);
+        hipchk(c, hipGetLastError(), "edge");
+        hipchk(c, hipMemcpyAsync(raw, draw, count * sizeof(double), hipMemcpyDeviceToHost, c->stream), "raw");
+        hipchk(c, hipMemcpyAsync(norm, dnorm, count * sizeof(double), hipMemcpyDeviceToHost, c->stream), "norm");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int m1lya_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc_r_right, int bc_z_bot,
+               int bc_z_top, double out[8])
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_r_right, "BC_r_right");
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        check_bc(c, bc_z_top, "BC_Z_top");
+        if (!(c_tilde > 0.0) || !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde and dt must be > 0");
+        fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, nullptr);
+        if (out) {
+            hipchk(c, hipMemcpyAsync(out, c->scalars, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                   "scalars");
+            check_err_flag(c, "step");
+        }
+    });
+}
+
+int m1lya_run_steps(m1lya_ctx* c, int n_steps, double c_tilde, double dt, double z_center, int bc_r_right,
+                    int bc_z_bot, int bc_z_top, double* history)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_r_right, "BC_r_right");
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        check_bc(c, bc_z_top, "BC_Z_top");
+        if (n_steps < 0) fail(c, M1LYA_ERR_ARG, "n_steps < 0");
+        if (!(c_tilde > 0.0) || !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde and dt must be > 0");
+        if (history && (size_t)n_steps * 8 > c->history_cap) {
+            if (c->history) hipchk(c, hipFree(c->history), "hipFree");
+            c->history = nullptr;
+            hipchk(c, hipMalloc(&c->history, (size_t)n_steps * 8 * sizeof(double)), "hipMalloc");
+            c->history_cap = (size_t)n_steps * 8;
+        }
+        for (int s = 0; s < n_steps; ++s)
+            fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top,
+                       history ? c->history + (size_t)s * 8 : nullptr);
+        if (history && n_steps > 0)
+            hipchk(c, hipMemcpyAsync(history, c->history, (size_t)n_steps * 8 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                   "history");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        check_err_flag(c, "run_steps");
+    });
+}
+
+int m1lya_set_profiling(m1lya_ctx* c, int on)
+{
+    return guarded(c, [&] {
+        collect_stats(c);
+        c->profiling = on != 0;
+    });
+}
+
+int m1lya_kernel_stats_count(m1lya_ctx* c)
+{
+    if (!c) return 0;
+    int n = 0;
+    int rc = guarded(c, [&] {
+        collect_stats(c);
+        n = (int)c->stats.size();
+    });
+    return rc == M1LYA_OK ? n : rc;
+}
+
+int m1lya_kernel_stats(m1lya_ctx* c, int index, char* name, size_t name_len, long* launches, double* total_ms)
+{
+    return guarded(c, [&] {
+        collect_stats(c);
+        if (index < 0 || index >= (int)c->stats.size()) fail(c, M1LYA_ERR_ARG, "stats index out of range");
+        auto it = c->stats.begin();
+        std::advance(it, index);
+        if (name && name_len) std::snprintf(name, name_len, "%s", it->first.c_str());
+        if (launches) *launches = it->second.launches;
+        if (total_ms) *total_ms = it->second.total_ms;
+    });
+}
+
+int m1lya_reset_kernel_stats(m1lya_ctx* c)
+{
+    return guarded(c, [&] {
+        collect_stats(c);
+        c->stats.clear();
+    });
+}
+
+int m1lya_selftest_math(int op, const double* a, const double* b, double* out, size_t n)
+{
+    if (!a || !out || (op == 0 && !b) || op < 0 || op > 2) {
+        g_last_error = "selftest: bad arguments";
+        return M1LYA_ERR_ARG;
+    }
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(double);
+    hipError_t e = hipMalloc(&da, bytes);
+    if (e == hipSuccess) e = hipMalloc(&db, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dout, bytes);
+    if (e == hipSuccess) e = hipMemcpy(da, a, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && b) e = hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n > 0) {
+        launch_selftest(nullptr, op, da, db, dout, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) {
+        g_last_error = std::string("selftest: ") + hipGetErrorString(e);
+        return M1LYA_ERR_HIP;
+    }
+    return M1LYA_OK;
+}
+
+}  // extern "C"

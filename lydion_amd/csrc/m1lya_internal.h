@@ -1,0 +1,139 @@
+// m1lya_internal.h -- kernel argument blocks and launch wrappers shared by
+// the kernels (m1lya_kernels.hip) and the C ABI (m1lya_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "m1lya_device.h"
+
+namespace m1 {
+
+// Cell columns per wavefront in the column kernel (lane-per-column dgtsv).
+constexpr int kColumnW = 4;
+
+struct DevGeom {
+    int nr, nz;
+    const double* A_r;        // [nz][nr+1]
+    const double* A_z;        // [nz+1][nr]
+    const double* inv_dV;     // [nz][nr]
+    const double* inv_r_cell; // [nr]
+    const double* ivr;        // [(nr+1)][5] interval set of every r interface
+    const double* ivz;        // [(nz+1)][5] interval set of every Z interface
+};
+
+struct DevVel {
+    const double* ur_star;    // [nz][nr+1]
+    const double* uz_star;    // [nz+1][nr]
+    const double* dur_dr;
+    const double* dur_dz;
+    const double* duz_dr;
+    const double* duz_dz;
+    const double* ur_over_r;
+};
+
+struct AdvectArgs {
+    int nn;
+    DevGeom g;
+    DevVel v;
+    const double* J;
+    const double* Hr;
+    const double* Hz;
+    const double* aF;
+    double* J_out;
+    double* Hr_out;
+    double* Hz_out;
+    double cdt;
+    int bc_r_right, bc_z_bot, bc_z_top, hll;
+    const int* krange;
+};
+
+enum ColumnFlags {
+    COL_EMIT = 1,     // LyαEmissionAbsorptionUpdate!
+    COL_FREQ = 2,     // frequency factors computed in-kernel
+    COL_STORE_F = 4,  // ... and stored to Fij_kp/k/km
+    COL_SOLVE = 8,    // tridiagonal frequency solve
+    COL_HCORR = 16,   // J floor + H rescale + causality
+    COL_FUSED = 32    // fused step: no *_old arrays, per-block reductions
+};
+
+struct ColumnArgs {
+    int nn, nr, nz, flags;
+    double* J;
+    double* Hr;
+    double* Hz;
+    double* J_old;
+    double* Hr_old;
+    double* Hz_old;
+    const double* Hv;
+    const double* phi;
+    const double* aF;
+    double* Fkp;
+    double* Fk;
+    double* Fkm;
+    const double* alpha;
+    const double* alpha_c;
+    const double* pd;
+    const double* js;
+    const double* T;
+    const double* dnuD;
+    const double* dur_dr;
+    const double* dur_dz;
+    const double* duz_dr;
+    const double* duz_dz;
+    const double* ur_over_r;
+    double* drhou_r;
+    double* drhou_z;
+    double* f_r;
+    double* f_z;
+    const double* nu;
+    const double* nuC;
+    const double* dnu;
+    const double* inv_dnu;
+    const double* inv_nuC;
+    const double* dV;
+    const double* rcom;
+    const double* zcom;
+    const double* A_r;
+    const double* A_z;
+    double c_tilde, dt, dw, alpha_floor, z_center;
+    int recoil, bc_z_bot;
+    double* part;
+    int pstride;
+    int* err_flag;
+};
+
+struct MfArgs {
+    int nr, nz;
+    const double* drhou_r;
+    const double* drhou_z;
+    double* f_r;
+    double* f_z;
+    const double* js;
+    const double* dV;
+    const double* rcom;
+    const double* zcom;
+    const double* fg_r;   // may be null
+    const double* fg_z;
+    double dt, z_center;
+    double* part;         // [nblocks][8]
+};
+
+void launch_active_partial(hipStream_t s, const double* J, const double* dnu, const double* dV, int nn,
+                           int ncells, int nblocks, double* part, int pstride);
+void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out);
+void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange);
+void launch_advect(hipStream_t s, const AdvectArgs& a);
+int column_width();
+void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks);
+void launch_step_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int bc_z_bot,
+                       int* krange, double* scalars, double* history);
+void launch_mf_partial(hipStream_t s, const MfArgs& a, int nblocks);
+void launch_fesc_partial(hipStream_t s, const double* Hr, const double* Hz, const double* A_r,
+                         const double* A_z, const double* dnu, int nn, int nr, int nz, int bc_z_bot,
+                         int nblocks, double* part);
+void launch_e_lya(hipStream_t s, const double* J, const double* dnu, int nn, int ncells, double* e);
+void launch_edge(hipStream_t s, const double* J, const double* dnu, int nn, int nr, int nz, double* raw,
+                 double* norm);
+void launch_nonfinite(hipStream_t s, const double* x, size_t n, int* flag);
+void launch_selftest(hipStream_t s, int op, const double* a, const double* b, double* out, size_t n);
+
+}  // namespace m1

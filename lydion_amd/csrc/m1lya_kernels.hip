@@ -1,0 +1,838 @@
+// m1lya_kernels.hip -- HIP kernels of the FP64 M1 Lya step for gfx950.
+//
+// Layout: spectral arrays are [j][i][k] (k = frequency bin fastest), exactly the
+// Julia column-major [k,i,j] of the reference, so a wavefront of 64 lanes on
+// consecutive k reads 512 contiguous bytes per FP64 field.
+#include "m1lya_internal.h"
+
+namespace m1 {
+
+// ------------------------------------------------------------------------
+// Wave helpers (64-wide wavefronts)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------------
+// FindActiveBins partial sums: src/UpdateLyaJandH.jl:61-90.
+// Block b sums over its cell range, per bin k:
+//   e1[k]  = Σ Δν[k] * max(J - Jeps, 0) * ΔV
+//   e10[k] = Σ Δν[k] * max(J - 10 Jeps, 0) * ΔV
+// part[b][0..nn) = e1, part[b][nn..2nn) = e10 (row stride = pstride).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_active_partial(const double* __restrict__ J,
+                                                         const double* __restrict__ dnu,
+                                                         const double* __restrict__ dV, int nn,
+                                                         int ncells, int cells_per_block,
+                                                         double* __restrict__ part, int pstride)
+{
+    __shared__ double s1[256], s2[256];
+    const int kw = ((nn + 63) / 64) * 64 > 256 ? 256 : ((nn + 63) / 64) * 64;
+    const int t = threadIdx.x;
+    const int sub = t / kw, nsub = 256 / kw;
+    const int c0 = blockIdx.x * cells_per_block;
+    const int c1 = min(c0 + cells_per_block, ncells);
+    for (int kb = 0; kb < nn; kb += kw) {
+        const int k = kb + (t % kw);
+        double a1 = 0.0, a10 = 0.0;
+        if (k < nn && sub < nsub) {
+            const double d = dnu[k];
+            for (int c = c0 + sub; c < c1; c += nsub) {
+                double Jv = J[(size_t)c * nn + k];
+                double v = dV[c];
+                a1 += d * dmax(Jv - kJeps, 0.0) * v;
+                a10 += d * dmax(Jv - 10 * kJeps, 0.0) * v;
+            }
+        }
+        s1[t] = a1; s2[t] = a10;
+        __syncthreads();
+        if (t < kw && k < nn) {
+            double r1 = 0.0, r10 = 0.0;
+            for (int s = 0; s < nsub; ++s) { r1 += s1[s * kw + t]; r10 += s2[s * kw + t]; }
+            part[(size_t)blockIdx.x * pstride + k] = r1;
+            part[(size_t)blockIdx.x * pstride + nn + k] = r10;
+        }
+        __syncthreads();
+    }
+}
+
+// Column sums of a [nrows][stride] partial array: out[col] = Σ_rows part[row][col].
+// One block per column, fixed-shape tree -> deterministic.
+__global__ __launch_bounds__(256) void k_reduce_cols(const double* __restrict__ part, int nrows,
+                                                      int stride, double* __restrict__ out)
+{
+    __shared__ double s[256];
+    const int col = blockIdx.x;
+    double a = 0.0;
+    for (int r = threadIdx.x; r < nrows; r += 256) a += part[(size_t)r * stride + col];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[col] = s[0];
+}
+
+// Active-bin decision (src/UpdateLyaJandH.jl:73-96) from reduced e1/e10.
+__device__ void active_range(const double* red, int nn, int kcore_min, int kcore_max, int* krange)
+{
+    double Etot = 0.0;
+    for (int k = 0; k < nn; ++k) Etot += red[k];
+    int kmn = nn - 1, kmx = 0;
+    for (int k = 0; k < nn; ++k) {
+        if (red[nn + k] > kEpsActive * Etot) {
+            if (k < kmn) kmn = k;
+            if (k > kmx) kmx = k;
+        }
+    }
+    if (kcore_max > kmx) kmx = kcore_max;
+    if (kcore_min < kmn) kmn = kcore_min;
+    krange[0] = kmn;
+    krange[1] = kmx;
+}
+
+__global__ void k_active_final(const double* __restrict__ red, int nn, int kcore_min, int kcore_max,
+                               int* __restrict__ krange)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) active_range(red, nn, kcore_min, kcore_max, krange);
+}
+
+// ------------------------------------------------------------------------
+// Ghost-aware stencil loads for the r and Z sweeps.  The ghost values are the
+// ones the reference builds at each boundary:
+//   r axis (i < 0): mirror, H_r negated                  src/AdvectJandH.jl:666-673
+//   r outer (i >= N_r): :Reflect mirror / :Outflow copy with H_r = max(H_r,0)  :853-871
+//   Z bottom (j < 0): :Reflect mirror / :Outflow copy with H_Z = min(H_Z,0)    :1000-1018
+//   Z top (j >= N_Z): :Reflect mirror / :Outflow copy with H_Z = max(H_Z,0)    :1185-1203
+// ------------------------------------------------------------------------
+struct SpecIn {
+    const double* J;
+    const double* Hr;
+    const double* Hz;
+};
+
+__device__ __forceinline__ void load_r(const SpecIn& s, int nn, int nr, int j, int k, int c, int bc_right,
+                                       double& J, double& Hr, double& Hz)
+{
+    if (c < 0) {
+        size_t idx = ((size_t)j * nr + (-c - 1)) * nn + k;
+        J = s.J[idx]; Hr = -s.Hr[idx]; Hz = s.Hz[idx];
+    } else if (c < nr) {
+        size_t idx = ((size_t)j * nr + c) * nn + k;
+        J = s.J[idx]; Hr = s.Hr[idx]; Hz = s.Hz[idx];
+    } else if (bc_right == BC_REFLECT) {
+        size_t idx = ((size_t)j * nr + (2 * nr - 1 - c)) * nn + k;
+        J = s.J[idx]; Hr = -s.Hr[idx]; Hz = s.Hz[idx];
+    } else {
+        size_t idx = ((size_t)j * nr + (nr - 1)) * nn + k;
+        J = s.J[idx]; Hr = dmax(s.Hr[idx], 0.0); Hz = s.Hz[idx];
+    }
+}
+
+__device__ __forceinline__ void load_z(const SpecIn& s, int nn, int nr, int nz, int i, int k, int c,
+                                       int bc_bot, int bc_top, double& J, double& Hr, double& Hz)
+{
+    if (c < 0) {
+        if (bc_bot == BC_REFLECT) {
+            size_t idx = ((size_t)(-c - 1) * nr + i) * nn + k;
+            J = s.J[idx]; Hr = s.Hr[idx]; Hz = -s.Hz[idx];
+        } else {
+            size_t idx = (size_t)i * nn + k;
+            J = s.J[idx]; Hr = s.Hr[idx]; Hz = dmin(s.Hz[idx], 0.0);
+        }
+    } else if (c < nz) {
+        size_t idx = ((size_t)c * nr + i) * nn + k;
+        J = s.J[idx]; Hr = s.Hr[idx]; Hz = s.Hz[idx];
+    } else if (bc_top == BC_REFLECT) {
+        size_t idx = ((size_t)(2 * nz - 1 - c) * nr + i) * nn + k;
+        J = s.J[idx]; Hr = s.Hr[idx]; Hz = -s.Hz[idx];
+    } else {
+        size_t idx = ((size_t)(nz - 1) * nr + i) * nn + k;
+        J = s.J[idx]; Hr = s.Hr[idx]; Hz = dmax(s.Hz[idx], 0.0);
+    }
+}
+
+// ------------------------------------------------------------------------
+// AdvectJandH_Cylindrical!: src/AdvectJandH.jl:526-1390.
+// One thread per (k, cell); each thread evaluates the four interfaces of its
+// cell and applies the cell update (:1302-1382).  Reads the state, writes the
+// new state to separate buffers (all fluxes of a step are computed from the
+// same old state, as in the reference).  Bins outside [kmin, kmax] are copied.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advect(AdvectArgs a)
+{
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    const int j = blockIdx.z;
+    if (k >= nn || i >= nr) return;
+    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    const int kmin = a.krange[0], kmax = a.krange[1];
+    const SpecIn s{a.J, a.Hr, a.Hz};
+    if (k < kmin || k > kmax) {
+        a.J_out[idx] = s.J[idx];
+        a.Hr_out[idx] = s.Hr[idx];
+        a.Hz_out[idx] = s.Hz[idx];
+        return;
+    }
+    double FrL0 = 0.0, FrL1 = 0.0, FrL2 = 0.0, FrR0 = 0.0, FrR1 = 0.0, FrR2 = 0.0;
+    double FzL0 = 0.0, FzL1 = 0.0, FzL2 = 0.0, FzR0 = 0.0, FzR1 = 0.0, FzR2 = 0.0;
+    double kphi_im = 0.0, kphi_ip = 0.0;
+#pragma unroll 1
+    for (int side = 0; side < 4; ++side) {
+        Stencil st;
+        const double* iv;
+        double aFL, aFR, u;
+        int dir;
+        if (side < 2) {
+            dir = 0;
+            const int f = i + side;   // face index: left face of cell f
+            const int cl = f - 1;
+            const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) load_r(s, nn, nr, j, k, cells[m], a.bc_r_right, st.J[m], st.Hr[m], st.Hz[m]);
+            iv = a.g.ivr + (size_t)f * 5;
+            aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
+            aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
+            u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
+        } else {
+            dir = 1;
+            const int g = j + (side - 2);
+            const int cl = g - 1;
+            const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                load_z(s, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, st.J[m], st.Hr[m], st.Hz[m]);
+            iv = a.g.ivz + (size_t)g * 5;
+            aFL = a.aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
+            aFR = a.aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+            u = a.v.uz_star[(size_t)g * nr + i] * kInvC;
+        }
+        FaceResult r = interface_flux(dir, a.hll, st, iv, aFL, aFR, u);
+        if (side == 0) {
+            FrL0 = r.F.J; FrL1 = r.F.Hr; FrL2 = r.F.Hz;
+            kphi_im = (i == 0) ? kphi_inner_axis(r.J_R, r.Hz_R) : r.kphi_R;
+        } else if (side == 1) {
+            FrR0 = r.F.J; FrR1 = r.F.Hr; FrR2 = r.F.Hz;
+            kphi_ip = r.kphi_L;
+        } else if (side == 2) {
+            FzL0 = r.F.J; FzL1 = r.F.Hr; FzL2 = r.F.Hz;
+        } else {
+            FzR0 = r.F.J; FzR1 = r.F.Hr; FzR2 = r.F.Hz;
+        }
+    }
+
+    // ---- cell update: src/AdvectJandH.jl:1319-1379 ----
+    const int c = j * nr + i;
+    const double Ar_L = a.g.A_r[(size_t)j * (nr + 1) + i], Ar_R = a.g.A_r[(size_t)j * (nr + 1) + i + 1];
+    const double Az_L = a.g.A_z[(size_t)j * nr + i], Az_R = a.g.A_z[(size_t)(j + 1) * nr + i];
+    const double cdt = a.cdt;
+    const double cdt_dV = cdt * a.g.inv_dV[c];
+    const double cdt_over_c = cdt * kInvC;
+    const double durdr = a.v.dur_dr[c], durdz = a.v.dur_dz[c], duzdr = a.v.duz_dr[c];
+    const double duzdz = a.v.duz_dz[c], urr = a.v.ur_over_r[c];
+    const double Js = s.J[idx], Hrs = s.Hr[idx], Hzs = s.Hz[idx];
+    Edd D = D_edd(Hrs, Hzs, Js);
+    double Jn = Js - cdt_dV * ((Ar_R * FrR0 - Ar_L * FrL0) + (Az_R * FzR0 - Az_L * FzL0));
+    Jn = dmax(Jn, kJeps);
+    double Hzn = Hzs - cdt_dV * ((Ar_R * FrR2 - Ar_L * FrL2) + (Az_R * FzR2 - Az_L * FzL2));
+    double Hrn = Hrs - cdt_dV * ((Ar_R * FrR1 - Ar_L * FrL1) + (Az_R * FzR1 - Az_L * FzL1));
+    double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
+    double Kav = 0.5 * (kphi_im + kphi_ip);
+    Hrn += cdt * Kav * a.g.inv_r_cell[i];
+    double dHr = -cdt_over_c * (Hrs * durdr + Hzs * durdz);
+    double dHz = -cdt_over_c * (Hrs * duzdr + Hzs * duzdz);
+    Hrn = Hrn + dHr;
+    Hzn = Hzn + dHz;
+    Jn = dmax(Jn - cdt_over_c * Ddu * Js, kJeps);
+    // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
+    double Jc = dmax(Jn, kJeps);
+    double Hm2 = Hrn * Hrn + Hzn * Hzn;
+    if (Hm2 > Jc * Jc) {
+        double fac = Jc / sqrt(Hm2);
+        Hrn *= fac;
+        Hzn *= fac;
+    }
+    a.J_out[idx] = Jn;
+    a.Hr_out[idx] = Hrn;
+    a.Hz_out[idx] = Hzn;
+}
+
+// ------------------------------------------------------------------------
+// LAPACK dgtsv (NRHS = 1, published reference algorithm: elimination with
+// partial pivoting by row interchange) on one LDS-resident column.
+// Called by TridiagonalSolve, src/TridiagonalSolver.jl:54.
+// ------------------------------------------------------------------------
+__device__ int gtsv_col(int n, double* dl, double* d, double* du, double* b, int st)
+{
+    // element m of an array lives at [m * st]  (st = W: columns interleaved)
+#define DL(m) dl[(m) * st]
+#define D(m) d[(m) * st]
+#define DU(m) du[(m) * st]
+#define B(m) b[(m) * st]
+    for (int i = 0; i < n - 2; ++i) {
+        if (fabs(D(i)) >= fabs(DL(i))) {
+            if (D(i) != 0.0) {
+                double fact = DL(i) / D(i);
+                D(i + 1) = D(i + 1) - fact * DU(i);
+                B(i + 1) = B(i + 1) - fact * B(i);
+            } else {
+                return i + 1;
+            }
+            DL(i) = 0.0;
+        } else {
+            double fact = D(i) / DL(i);
+            D(i) = DL(i);
+            double temp = D(i + 1);
+            D(i + 1) = DU(i) - fact * temp;
+            DL(i) = DU(i + 1);
+            DU(i + 1) = -fact * DL(i);
+            DU(i) = temp;
+            temp = B(i);
+            B(i) = B(i + 1);
+            B(i + 1) = temp - fact * B(i + 1);
+        }
+    }
+    if (n > 1) {
+        int i = n - 2;
+        if (fabs(D(i)) >= fabs(DL(i))) {
+            if (D(i) != 0.0) {
+                double fact = DL(i) / D(i);
+                D(i + 1) = D(i + 1) - fact * DU(i);
+                B(i + 1) = B(i + 1) - fact * B(i);
+            } else {
+                return i + 1;
+            }
+        } else {
+            double fact = D(i) / DL(i);
+            D(i) = DL(i);
+            double temp = D(i + 1);
+            D(i + 1) = DU(i) - fact * temp;
+            DU(i) = temp;
+            temp = B(i);
+            B(i) = B(i + 1);
+            B(i + 1) = temp - fact * B(i + 1);
+        }
+    }
+    if (D(n - 1) == 0.0) return n;
+    B(n - 1) = B(n - 1) / D(n - 1);
+    if (n > 1) B(n - 2) = (B(n - 2) - DU(n - 2) * B(n - 1)) / D(n - 2);
+    for (int i = n - 3; i >= 0; --i) B(i) = (B(i) - DU(i) * B(i + 1) - DL(i) * B(i + 2)) / D(i);
+    return 0;
+#undef DL
+#undef D
+#undef DU
+#undef B
+}
+
+// ------------------------------------------------------------------------
+// Column kernel: one wavefront owns W cell columns at a time.
+//   P1 (lanes over k): emission/absorption (src/ConstantsAndFunctions.jl:483-573),
+//       frequency factors (src/FreqFactors.jl:142-278), tridiagonal rows to LDS
+//       (src/UpdateLyaJandH.jl:178-227);
+//   P2 (lane per column): dgtsv on the LDS column;
+//   P3 (lanes over k): J floor, H rescale and causality, old copies
+//       (src/UpdateLyaJandH.jl:335-379), plus the fused step's reductions.
+// LDS per block: 4 arrays x (N_ν-2) rows x W columns (column index fastest,
+// so the lane-per-column sweep is bank-conflict free).
+// ------------------------------------------------------------------------
+template <int W, int KPL>
+__global__ __launch_bounds__(64) void k_column(ColumnArgs a)
+{
+    extern __shared__ double lds[];
+    const int nn = a.nn, nr = a.nr, nz = a.nz;
+    const int n = nn - 2;
+    const int lane = threadIdx.x;
+    const int flags = a.flags;
+    const bool EMIT = flags & COL_EMIT, FREQ = flags & COL_FREQ, STOREF = flags & COL_STORE_F;
+    const bool SOLVE = flags & COL_SOLVE, HCORR = flags & COL_HCORR, FUSED = flags & COL_FUSED;
+    const size_t rows = (size_t)(n > 0 ? n : 1);
+    double* sDL = lds;
+    double* sD = lds + rows * W;
+    double* sDU = lds + 2 * rows * W;
+    double* sB = lds + 3 * rows * W;
+    const int ncells = nr * nz;
+    const int ngroups = (ncells + W - 1) / W;
+    const double ct = a.c_tilde, dt = a.dt, dw = a.dw;
+    const double inv_ct = 1.0 / ct;
+    const double fpc_ict = (kFourPi / kC) * inv_ct;
+    const double inv_dt = 1.0 / dt;
+
+    double e1[KPL], e10[KPL];
+#pragma unroll
+    for (int kk = 0; kk < KPL; ++kk) { e1[kk] = 0.0; e10[kk] = 0.0; }
+    double acc_mf = 0.0, acc_L = 0.0, acc_net = 0.0, acc_out = 0.0, acc_esc = 0.0;
+
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        double Jo[W][KPL], Hr[W][KPL], Hz[W][KPL];
+        // ================= P1 =================
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int c = grp * W + w;
+            if (c >= ncells) continue;
+            const size_t base = (size_t)c * nn;
+            double sr = 0.0, sz = 0.0;
+            double al_pd = 0.0, js_ct = 0.0, ac = 0.0;
+            if (EMIT) {
+                ac = a.alpha_c[c];
+                al_pd = a.alpha[c] * a.pd[c];
+                js_ct = a.js[c] * ct;
+            }
+            double Jcur[KPL];
+#pragma unroll
+            for (int kk = 0; kk < KPL; ++kk) {
+                const int k = lane + 64 * kk;
+                if (k >= nn) continue;
+                const size_t idx = base + k;
+                double jc = a.J[idx];
+                double jo = FUSED ? jc : a.J_old[idx];
+                double hr = a.Hr[idx], hz = a.Hz[idx];
+                if (EMIT) {
+                    // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462)
+                    double aJ = ac + al_pd * a.Hv[idx];
+                    double dJdt = js_ct * a.phi[idx];
+                    double jn = (jo + dJdt * dt) / (1.0 + aJ * ct * dt);
+                    double aH = a.aF[idx];
+                    double hrn = (hr + 0.0 * dt) / (1.0 + aH * ct * dt);
+                    double hzn = (hz + 0.0 * dt) / (1.0 + aH * ct * dt);
+                    const double dk = a.dnu[k];
+                    sr += dk * (hr - hrn);
+                    sz += dk * (hz - hzn);
+                    jo = jn; jc = jn; hr = hrn; hz = hzn;
+                }
+                Jo[w][kk] = jo; Hr[w][kk] = hr; Hz[w][kk] = hz; Jcur[kk] = jc;
+            }
+            if (EMIT) {
+                sr = wave_sum(sr);
+                sz = wave_sum(sz);
+                if (!FUSED) {
+                    if (lane == 0) {
+                        a.drhou_r[c] += fpc_ict * sr;
+                        a.drhou_z[c] += fpc_ict * sz;
+                    }
+                } else {
+                    // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
+                    const int i = c % nr, j = c / nr;
+                    double dr_u = a.drhou_r[c] + fpc_ict * sr;
+                    double dz_u = a.drhou_z[c] + fpc_ict * sz;
+                    double fr = dr_u * inv_dt, fz = dz_u * inv_dt;
+                    if (lane == 0) {
+                        a.f_r[c] = fr; a.f_z[c] = fz;
+                        a.drhou_r[c] = 0.0; a.drhou_z[c] = 0.0;
+                    }
+                    const double dV = a.dV[c];
+                    acc_mf += dV * sqrt(fr * fr + fz * fz);
+                    acc_L += kFourPi * dV * a.js[c];
+                    double dr = a.rcom[i], dz = a.zcom[j] - a.z_center;
+                    double dist = sqrt(dr * dr + dz * dz);
+                    double frad = 0.0;
+                    if (dist > 0.0) {
+                        double inv_d = 1.0 / dist;
+                        frad = fr * (dr * inv_d) + fz * (dz * inv_d);
+                    }
+                    acc_net += dV * frad;
+                    acc_out += dV * dmax(frad, 0.0);
+                }
+            }
+            if (FREQ || SOLVE) {
+                const double dnuD = a.dnuD ? a.dnuD[c] : 0.0;
+                const double PREFACT = FREQ ? 0.5 * dmax(a.alpha[c], a.alpha_floor) * (dnuD * dnuD) : 0.0;
+                const double RECOIL_0 = FREQ ? kHoverKB / a.T[c] : 0.0;
+                const double durdr = FREQ ? a.dur_dr[c] : 0.0, durdz = FREQ ? a.dur_dz[c] : 0.0;
+                const double duzdr = FREQ ? a.duz_dr[c] : 0.0, duzdz = FREQ ? a.duz_dz[c] : 0.0;
+                const double urr = FREQ ? a.ur_over_r[c] : 0.0;
+#pragma unroll
+                for (int kk = 0; kk < KPL; ++kk) {
+                    const int k = lane + 64 * kk;
+                    if (k < 1 || k > nn - 2) continue;
+                    const size_t idx = base + k;
+                    double Fkp, Fk, Fkm;
+                    if (FREQ) {
+                        const double* nu = a.nu;
+                        const double* nuC = a.nuC;
+                        double nuCk = nuC[k], nuCkp = nuC[k + 1], nuCkm = nuC[k - 1];
+                        double dkp = nuCkp - nu[k + 1];
+                        double dk = nu[k + 1] - nuCk;
+                        double dkm = nu[k] - nuCkm;
+                        double inv_dnu_k = a.inv_dnu[k];
+                        double two_k = 2.0 * a.inv_nuC[k];
+                        double two_kp = 2.0 * a.inv_nuC[k + 1];
+                        double two_km = 2.0 * a.inv_nuC[k - 1];
+                        Edd D = D_edd(Hr[w][kk], Hz[w][kk], Jcur[kk]);
+                        double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
+                        double VEL = Ddu * kInvC;
+                        double Kkp = PREFACT * a.Hv[idx + 1], Kk = PREFACT * a.Hv[idx];
+                        double Kkm = PREFACT * a.Hv[idx - 1];
+                        double Ckp = VEL * nuCkp, Ck = VEL * nuCk, Ckm = VEL * nuCkm;
+                        if (a.recoil) {
+                            Ckp += Kkp * (RECOIL_0 - two_kp);
+                            Ck += Kk * (RECOIL_0 - two_k);
+                            Ckm += Kkm * (RECOIL_0 - two_km);
+                        }
+                        double qkp = dkp * (1.0 / Kkp), qk = dk * (1.0 / Kk), qkm = dkm * (1.0 / Kkm);
+                        double Pkp = dclamp(Ckp * qkp, -kPmax, kPmax);
+                        double Pk = dclamp(Ck * qk, -kPmax, kPmax);
+                        double Pkm = dclamp(Ckm * qkm, -kPmax, kPmax);
+                        double oRkp, xRkp, oLk, xLk, oRk, xRk, oLkm, xLkm;
+                        one_minus_phi_R(Pkp, oRkp, xRkp);
+                        one_minus_phi_L(Pk, oLk, xLk);
+                        one_minus_phi_R(Pk, oRk, xRk);
+                        one_minus_phi_L(Pkm, oLkm, xLkm);
+                        double den_kp = inv_dnu_k * (1.0 / (xLk * qk - xRkp * qkp));
+                        Fkp = oRkp * den_kp;
+                        double den_km = inv_dnu_k * (1.0 / (xLkm * qkm - xRk * qk));
+                        Fkm = oLkm * den_km;
+                        Fk = oLk * den_kp + oRk * den_km;
+                        if (STOREF) { a.Fkp[idx] = Fkp; a.Fk[idx] = Fk; a.Fkm[idx] = Fkm; }
+                    } else {
+                        Fkp = a.Fkp[idx]; Fk = a.Fk[idx]; Fkm = a.Fkm[idx];
+                    }
+                    if (SOLVE) {
+                        // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215) in gtsv form:
+                        // dl <- A[2:end], du <- C[1:end-1] (src/TridiagonalSolver.jl:45-46)
+                        const int row = k - 1;
+                        sD[(size_t)row * W + w] = -(Fk + dw);
+                        sB[(size_t)row * W + w] = -dw * Jo[w][kk];
+                        if (row >= 1) sDL[(size_t)(row - 1) * W + w] = Fkm;
+                        if (row <= n - 2) sDU[(size_t)row * W + w] = Fkp;
+                    }
+                }
+            }
+        }
+        // ================= P2 =================
+        if (SOLVE) {
+            __syncthreads();
+            const int c = grp * W + lane;
+            if (lane < W && c < ncells && n > 0) {
+                int info = gtsv_col(n, sDL + lane, sD + lane, sDU + lane, sB + lane, W);
+                if (info != 0) atomicMax(a.err_flag, 1);
+            }
+            __syncthreads();
+        }
+        // ================= P3 =================
+        if (!(EMIT || SOLVE || HCORR)) continue;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int c = grp * W + w;
+            if (c >= ncells) continue;
+            const size_t base = (size_t)c * nn;
+            const int i = c % nr, j = c / nr;
+            const double dV = a.dV ? a.dV[c] : 0.0;
+            double esc = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < KPL; ++kk) {
+                const int k = lane + 64 * kk;
+                if (k >= nn) continue;
+                const size_t idx = base + k;
+                double jn = Jo[w][kk];
+                double hr = Hr[w][kk], hz = Hz[w][kk];
+                if (SOLVE) {
+                    if (k >= 1 && k <= nn - 2) jn = sB[(size_t)(k - 1) * W + w];
+                    else if (!FUSED) jn = a.J[idx];
+                }
+                if (HCORR) {
+                    double Jk = dmax(jn, kJeps);
+                    double inv_Jold = 1.0 / dmax(Jo[w][kk], kJeps);
+                    double ratio = Jk * inv_Jold;
+                    hr = hr * ratio;
+                    hz = hz * ratio;
+                    double Hm2 = hr * hr + hz * hz;
+                    if (Hm2 > Jk * Jk) {
+                        double s = Jk / sqrt(Hm2);
+                        hr = hr * s;
+                        hz = hz * s;
+                    }
+                    jn = Jk;
+                }
+                a.J[idx] = jn; a.Hr[idx] = hr; a.Hz[idx] = hz;
+                if (!FUSED) { a.J_old[idx] = jn; a.Hr_old[idx] = hr; a.Hz_old[idx] = hz; }
+                if (FUSED) {
+                    const double dk = a.dnu[k];
+                    e1[kk] += dk * dmax(jn - kJeps, 0.0) * dV;
+                    e10[kk] += dk * dmax(jn - 10 * kJeps, 0.0) * dV;
+                    // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
+                    if (i == nr - 1) esc += dk * (a.A_r[(size_t)j * (nr + 1) + nr] * kFourPi * dmax(hr, 0.0));
+                    if (j == nz - 1) esc += dk * (a.A_z[(size_t)nz * nr + i] * kFourPi * dmax(hz, 0.0));
+                    if (j == 0 && a.bc_z_bot == BC_OUTFLOW) esc += dk * (a.A_z[i] * kFourPi * dmax(-hz, 0.0));
+                }
+            }
+            if (FUSED && (i == nr - 1 || j == nz - 1 || (j == 0 && a.bc_z_bot == BC_OUTFLOW)))
+                acc_esc += wave_sum(esc);
+        }
+    }
+    if (FUSED) {
+        double* prow = a.part + (size_t)blockIdx.x * a.pstride;
+#pragma unroll
+        for (int kk = 0; kk < KPL; ++kk) {
+            const int k = lane + 64 * kk;
+            if (k < nn) { prow[k] = e1[kk]; prow[nn + k] = e10[kk]; }
+        }
+        if (lane == 0) {
+            prow[2 * nn + 0] = acc_mf;
+            prow[2 * nn + 1] = acc_L;
+            prow[2 * nn + 2] = acc_net;
+            prow[2 * nn + 3] = acc_out;
+            prow[2 * nn + 4] = acc_esc;
+        }
+    }
+}
+
+// Step scalars from the reduced partials of the fused column kernel:
+// Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673),
+// Compute_f_esc_Lyα_Cylindrical (:969-993) and FindActiveBins for the next step.
+__global__ void k_step_final(const double* __restrict__ red, int nn, int kcore_min, int kcore_max,
+                             int bc_z_bot, int* __restrict__ krange, double* __restrict__ scalars,
+                             double* __restrict__ history)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int kmin_used = krange[0], kmax_used = krange[1];
+    double MF = red[2 * nn + 0], L = red[2 * nn + 1], net = red[2 * nn + 2], out = red[2 * nn + 3];
+    double esc = red[2 * nn + 4];
+    if (L > 0.0) {
+        double s = 1.0 / (L / kC);
+        MF = MF * s; net = net * s; out = out * s;
+    } else {
+        MF = 0.0; net = 0.0; out = 0.0;
+    }
+    double Ltot = (bc_z_bot == BC_REFLECT) ? 2.0 * L : L;
+    double fesc = 0.0;
+    if (Ltot > 0.0) {
+        double Lsim = (bc_z_bot == BC_REFLECT) ? 0.5 * Ltot : Ltot;
+        fesc = (Lsim > 0.0) ? esc / Lsim : 0.0;
+    }
+    double v[8] = {MF, net, out, Ltot, fesc, (double)kmin_used, (double)kmax_used, 0.0};
+    for (int m = 0; m < 8; ++m) scalars[m] = v[m];
+    if (history)
+        for (int m = 0; m < 8; ++m) history[m] = v[m];
+    active_range(red, nn, kcore_min, kcore_max, krange);
+}
+
+// ------------------------------------------------------------------------
+// Operator-level reductions
+// ------------------------------------------------------------------------
+// Update_f_Lyα_MF_L_Lyα! (and the gravity variant), per-block partials:
+// part[b] = {Σ ΔV|f|, Σ 4π ΔV js, Σ ΔV f_rad, Σ ΔV max(f_rad,0), Σ ΔV f_antigrav}
+__global__ __launch_bounds__(256) void k_mf_partial(MfArgs a)
+{
+    __shared__ double s[5][256];
+    const int nr = a.nr, nz = a.nz, ncells = nr * nz;
+    double acc[5] = {0, 0, 0, 0, 0};
+    const double inv_dt = 1.0 / a.dt;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) {
+        const int i = c % nr, j = c / nr;
+        double fr = a.drhou_r[c] * inv_dt, fz = a.drhou_z[c] * inv_dt;
+        a.f_r[c] = fr; a.f_z[c] = fz;
+        const double dV = a.dV[c];
+        acc[0] += dV * sqrt(fr * fr + fz * fz);
+        acc[1] += kFourPi * dV * a.js[c];
+        double dr = a.rcom[i], dz = a.zcom[j] - a.z_center;
+        double dist = sqrt(dr * dr + dz * dz);
+        double frad = 0.0;
+        if (dist > 0.0) {
+            double inv_d = 1.0 / dist;
+            frad = fr * (dr * inv_d) + fz * (dz * inv_d);
+        }
+        acc[2] += dV * frad;
+        acc[3] += dV * dmax(frad, 0.0);
+        if (a.fg_r) {
+            double gr = a.fg_r[c], gz = a.fg_z[c];
+            double gm = sqrt(gr * gr + gz * gz);
+            double fa = (gm > 0.0) ? -(fr * gr + fz * gz) / gm : 0.0;
+            acc[4] += dV * fa;
+        }
+    }
+    for (int m = 0; m < 5; ++m) s[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int m = 0; m < 5; ++m) s[m][threadIdx.x] += s[m][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 5; ++m) a.part[(size_t)blockIdx.x * 8 + m] = s[m][0];
+}
+
+// Compute_f_esc_Cylindrical leak sums, per block: Σ_k Δν_k Σ_faces A 4π max(±H, 0).
+__global__ __launch_bounds__(256) void k_fesc_partial(const double* __restrict__ Hr,
+                                                       const double* __restrict__ Hz,
+                                                       const double* __restrict__ A_r,
+                                                       const double* __restrict__ A_z,
+                                                       const double* __restrict__ dnu, int nn, int nr,
+                                                       int nz, int bc_z_bot, double* __restrict__ part)
+{
+    __shared__ double s[256];
+    const int nfaces = nz + nr + (bc_z_bot == BC_OUTFLOW ? nr : 0);
+    const long total = (long)nfaces * nn;
+    double acc = 0.0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long)gridDim.x * 256) {
+        const int k = (int)(t % nn);
+        const int f = (int)(t / nn);
+        double term;
+        if (f < nz) {
+            term = A_r[(size_t)f * (nr + 1) + nr] * kFourPi * dmax(Hr[((size_t)f * nr + nr - 1) * nn + k], 0.0);
+        } else if (f < nz + nr) {
+            const int i = f - nz;
+            term = A_z[(size_t)nz * nr + i] * kFourPi * dmax(Hz[((size_t)(nz - 1) * nr + i) * nn + k], 0.0);
+        } else {
+            const int i = f - nz - nr;
+            term = A_z[i] * kFourPi * dmax(-Hz[(size_t)i * nn + k], 0.0);
+        }
+        acc += dnu[k] * term;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Update_e_Lyα!: one wavefront per cell, e = (4π/c) Σ_k Δν_k J_k.
+__global__ __launch_bounds__(256) void k_e_lya(const double* __restrict__ J, const double* __restrict__ dnu,
+                                                int nn, int ncells, double* __restrict__ e)
+{
+    const int c = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (c >= ncells) return;
+    double s = 0.0;
+    for (int k = lane; k < nn; k += 64) s += dnu[k] * J[(size_t)c * nn + k];
+    s = wave_sum(s);
+    if (lane == 0) e[c] = (kFourPi / kC) * s;
+}
+
+// Edge spectra: raw J of the edge cells and J / Σ_k J_k Δν_k.
+__global__ __launch_bounds__(64) void k_edge(const double* __restrict__ J, const double* __restrict__ dnu,
+                                              int nn, int nr, int nz, double* __restrict__ raw,
+                                              double* __restrict__ norm)
+{
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x;
+    int i, j;
+    const int nright = nz - 2, ntop = nr - 3;
+    if (e < nright) { i = nr - 2; j = 1 + e; }
+    else if (e < nright + ntop) { i = 1 + (e - nright); j = nz - 2; }
+    else { i = 1 + (e - nright - ntop); j = 1; }
+    const double* Jc = J + ((size_t)j * nr + i) * nn;
+    double s = 0.0;
+    for (int k = lane; k < nn; k += 64) s += Jc[k] * dnu[k];
+    s = wave_sum(s);
+    for (int k = lane; k < nn; k += 64) {
+        raw[(size_t)e * nn + k] = Jc[k];
+        norm[(size_t)e * nn + k] = (s > 0.0) ? Jc[k] / s : Jc[k];
+    }
+}
+
+// Non-finite scan of a state array: sets *flag when a NaN/Inf is found.
+__global__ __launch_bounds__(256) void k_nonfinite(const double* __restrict__ x, size_t n, int* flag)
+{
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) {
+        double v = x[t];
+        if (!(v - v == 0.0)) atomicMax(flag, 1);
+    }
+}
+
+__global__ void k_selftest(int op, const double* a, const double* b, double* out, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (op == 0) out[t] = a[t] / b[t];
+    else if (op == 1) out[t] = sqrt(a[t]);
+    else out[t] = exp_det(a[t]);
+}
+
+// ------------------------------------------------------------------------
+// Launch wrappers
+// ------------------------------------------------------------------------
+void launch_active_partial(hipStream_t s, const double* J, const double* dnu, const double* dV, int nn,
+                           int ncells, int nblocks, double* part, int pstride)
+{
+    int cpb = (ncells + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(k_active_partial, dim3(nblocks), dim3(256), 0, s, J, dnu, dV, nn, ncells, cpb, part,
+                       pstride);
+}
+
+void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out)
+{
+    hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(256), 0, s, part, nrows, stride, out);
+}
+
+void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange)
+{
+    hipLaunchKernelGGL(k_active_final, dim3(1), dim3(64), 0, s, red, nn, kcmin, kcmax, krange);
+}
+
+void launch_advect(hipStream_t s, const AdvectArgs& a)
+{
+    dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
+    hipLaunchKernelGGL(k_advect, grid, dim3(64, 4), 0, s, a);
+}
+
+template <int W, int KPL>
+static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
+{
+    size_t rows = (size_t)(a.nn - 2 > 0 ? a.nn - 2 : 1);
+    size_t lds = 4 * rows * W * sizeof(double);
+    hipLaunchKernelGGL((k_column<W, KPL>), dim3(nblocks), dim3(64), lds, s, a);
+}
+
+int column_width() { return kColumnW; }
+
+void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks)
+{
+    const int kpl = (a.nn + 63) / 64;
+    switch (kpl) {
+    case 1: launch_column_t<kColumnW, 1>(s, a, nblocks); break;
+    case 2: launch_column_t<kColumnW, 2>(s, a, nblocks); break;
+    case 3: launch_column_t<kColumnW, 3>(s, a, nblocks); break;
+    default: launch_column_t<kColumnW, 4>(s, a, nblocks); break;
+    }
+}
+
+void launch_step_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int bc_z_bot,
+                       int* krange, double* scalars, double* history)
+{
+    hipLaunchKernelGGL(k_step_final, dim3(1), dim3(64), 0, s, red, nn, kcmin, kcmax, bc_z_bot, krange, scalars,
+                       history);
+}
+
+void launch_mf_partial(hipStream_t s, const MfArgs& a, int nblocks)
+{
+    hipLaunchKernelGGL(k_mf_partial, dim3(nblocks), dim3(256), 0, s, a);
+}
+
+void launch_fesc_partial(hipStream_t s, const double* Hr, const double* Hz, const double* A_r,
+                         const double* A_z, const double* dnu, int nn, int nr, int nz, int bc_z_bot,
+                         int nblocks, double* part)
+{
+    hipLaunchKernelGGL(k_fesc_partial, dim3(nblocks), dim3(256), 0, s, Hr, Hz, A_r, A_z, dnu, nn, nr, nz,
+                       bc_z_bot, part);
+}
+
+void launch_e_lya(hipStream_t s, const double* J, const double* dnu, int nn, int ncells, double* e)
+{
+    hipLaunchKernelGGL(k_e_lya, dim3((ncells + 3) / 4), dim3(256), 0, s, J, dnu, nn, ncells, e);
+}
+
+void launch_edge(hipStream_t s, const double* J, const double* dnu, int nn, int nr, int nz, double* raw,
+                 double* norm)
+{
+    int ne = (nz - 2) + 2 * (nr - 3);
+    if (ne > 0) hipLaunchKernelGGL(k_edge, dim3(ne), dim3(64), 0, s, J, dnu, nn, nr, nz, raw, norm);
+}
+
+void launch_nonfinite(hipStream_t s, const double* x, size_t n, int* flag)
+{
+    hipLaunchKernelGGL(k_nonfinite, dim3(1024), dim3(256), 0, s, x, n, flag);
+}
+
+void launch_selftest(hipStream_t s, int op, const double* a, const double* b, double* out, size_t n)
+{
+    hipLaunchKernelGGL(k_selftest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, op, a, b, out, n);
+}
+
+}  // namespace m1

@@ -1,0 +1,275 @@
+"""Frequency grid, Voigt profile and the LyαArrays host container of the Lya
+M1 step (src/SetupLymanAlphaArrays.jl:17-393, src/ConstantsAndFunctions.jl:219-393,
+src/FastInterpolations.jl:26-274).
+
+These are the setup inputs of the hot path.  The host computes them once per
+run (static clouds) and uploads them; the step itself runs in libm1lya.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.special import dawsn
+
+from . import constants as K
+
+
+# ---- per-temperature line parameters: src/ConstantsAndFunctions.jl:282-314 ----
+def compute_b(T):
+    return K.b_PREFACT * np.sqrt(T)
+
+
+def compute_dnuD(T):
+    return compute_b(T) * K.nu_Lya_over_c
+
+
+def compute_a(T):
+    return K.a_PREFACT / compute_b(T)
+
+
+def compute_sigma0(T):
+    return K.sigma0_PREFACT / compute_b(T)
+
+
+# ---- frequency grids ---------------------------------------------------------
+def reference_freq_grid(Tmin: float = 30.0, q_fine: float = 1.1, q_wing: float | None = None) -> np.ndarray:
+    """ν interfaces of GenerateLyαArrays (src/SetupLymanAlphaArrays.jl:40-100)."""
+    if q_wing is None:
+        q_wing = q_fine
+    assert q_fine >= 1.0 and q_wing >= 1.0
+    x_fine_max_3e4 = 5.0
+    x_max = 1e5
+    dnu_min = (compute_b(Tmin) / K.c) * K.nu_Lya
+    x_max = min(x_max, 0.99 * K.nu_Lya / dnu_min)
+    x_max = min(x_max, (K.nu_LyC - K.nu_Lya) / dnu_min)
+    xfm = x_fine_max_3e4 * compute_b(3e4) / compute_b(Tmin)
+    x_pos = [0.0]
+    dx = 1.0
+    while x_pos[-1] < x_max:
+        x = x_pos[-1]
+        y = (x - xfm) / (0.1 * xfm)
+        S = 0.0 if y < 0.0 else (1.0 if y > 1.0 else y)
+        q = q_fine + (q_wing - q_fine) * S
+        x_pos.append(x_pos[-1] + dx)
+        dx *= q
+    if x_pos[-1] > x_max:
+        x_pos[-1] = x_max
+    elif x_pos[-1] < x_max:
+        x_pos.append(x_max)
+    x_pos = np.array(x_pos)
+    x_all = np.concatenate([-x_pos[1:][::-1], x_pos])
+    return K.nu_Lya + x_all * dnu_min
+
+
+def bins_freq_grid(n_nu: int, x_max_doppler: float, T_ref: float, Tmin: float = 30.0) -> np.ndarray:
+    """ν interfaces with exactly n_nu bins spanning ±x_max_doppler·ΔνD(T_ref).
+
+    The reference's geometric rule (src/SetupLymanAlphaArrays.jl:55-93): symmetric
+    about the line centre, first bin one Doppler width at Tmin, widths growing by a
+    factor q.  Here q is solved so that the positive side has exactly n_nu/2 bins,
+    the last one clipped at x_max.  When x_max is narrower than n_nu/2 Tmin
+    Doppler widths, the bins are uniform.  The result is passed to the step the way
+    GenerateLyαArrays_FromSaved (:244-393) accepts an arbitrary grid."""
+    if n_nu % 2 or n_nu < 4:
+        raise ValueError("n_nu must be even and >= 4")
+    n = n_nu // 2
+    dnu_min = (compute_b(Tmin) / K.c) * K.nu_Lya
+    x_max = x_max_doppler * compute_dnuD(T_ref) / dnu_min
+    if x_max / n <= 1.0:
+        dx0, q = x_max / n, 1.0
+    else:
+        dx0 = 1.0
+        lo, hi = 1.0, 4.0
+        s = lambda q, m: m if q == 1.0 else (q ** m - 1.0) / (q - 1.0)  # noqa: E731
+        for _ in range(200):
+            q = 0.5 * (lo + hi)
+            if s(q, n - 0.5) < x_max:
+                lo = q
+            else:
+                hi = q
+    x_pos = [0.0]
+    dx = dx0
+    for _ in range(1, n):
+        x_pos.append(x_pos[-1] + dx)
+        dx *= q
+    x_pos.append(x_max)
+    x_pos = np.array(x_pos)
+    if not np.all(np.diff(x_pos) > 0):
+        raise ValueError("frequency grid is not increasing")
+    x_all = np.concatenate([-x_pos[1:][::-1], x_pos])
+    return K.nu_Lya + x_all * dnu_min
+
+
+def core_range(nuC: np.ndarray) -> tuple[int, int]:
+    """kcore_min/kcore_max (0-based): bins with |νC - νLyα| <= 4 ΔνD(3e4 K)
+    (src/SetupLymanAlphaArrays.jl:133-135)."""
+    dnu_D3e4 = (compute_b(3e4) / K.c) * K.nu_Lya
+    lo = np.nonzero(nuC >= K.nu_Lya - 4 * dnu_D3e4)[0]
+    hi = np.nonzero(nuC <= K.nu_Lya + 4 * dnu_D3e4)[0]
+    return int(lo[0]), int(hi[-1])
+
+
+# ---- Voigt profile: src/FastInterpolations.jl:26-274, src/ConstantsAndFunctions.jl:219-357 ----
+XMAX_VOIGT = 1000.0
+NX_VOIGT = 200001
+
+
+def voigt_luts(xmax: float = XMAX_VOIGT, nx: int = NX_VOIGT) -> np.ndarray:
+    """CreateVoigtLUTs: node values of erf, dawson, exp(-x²), x exp(-x²), (1-2x²) exp(-x²)
+    on x_i = xmin + i Δx, Δx = (xmax - xmin)/(nx - 1).  Shape (5, nx).
+    erf uses the FDLIBM-derived libm erf; dawson the Faddeeva package (scipy.special.dawsn),
+    the implementation SpecialFunctions.jl wraps."""
+    xmin = -xmax
+    dx = (xmax - xmin) / (nx - 1)
+    x = xmin + np.arange(nx, dtype=np.float64) * dx
+    g = np.exp(-x * x)
+    erf = np.array([math.erf(v) for v in x])
+    return np.stack([erf, dawsn(x), g, x * g, (1.0 - 2.0 * (x * x)) * g])
+
+
+def _lut_eval(y: np.ndarray, nx: int, xmin: float, inv_dx: float, x: np.ndarray) -> np.ndarray:
+    """EvaluateLUT (src/FastInterpolations.jl:83-112)."""
+    xt = (x - xmin) * inv_dx
+    i = np.floor(xt).astype(np.int64)
+    i = np.where(i + 2 > nx, nx - 2, i)
+    d = (i + 1).astype(np.float64) - xt
+    return d * y[i] + (1.0 - d) * y[i + 1]
+
+
+class VoigtLUT:
+    def __init__(self, luts: np.ndarray | None = None, xmax: float = XMAX_VOIGT):
+        self.t = voigt_luts(xmax) if luts is None else luts
+        self.nx = self.t.shape[1]
+        self.xmax = xmax
+        self.xmin = -xmax
+        self.inv_dx = 1.0 / ((xmax - (-xmax)) / (self.nx - 1))
+
+    def _fn(self, which: int, x: np.ndarray) -> np.ndarray:
+        inside = np.abs(x) < self.xmax
+        xs = np.where(inside, x, 0.0)
+        v = _lut_eval(self.t[which], self.nx, self.xmin, self.inv_dx, xs)
+        if which == 0:
+            out = np.copysign(1.0, x)
+        elif which == 1:
+            with np.errstate(divide="ignore"):
+                ix = 1.0 / x
+            out = 0.5 * ix + 0.25 * (ix * ix * ix)
+        else:
+            out = np.zeros_like(x)
+        return np.where(inside, v, out)
+
+    def H(self, x, a):
+        """H(x,a), Smith+2018 eq. 12 (src/ConstantsAndFunctions.jl:227-239)."""
+        return self._fn(2, x) + K.inv_Sqrt_pi_over_2 * a * (2.0 * x * self._fn(1, x) - 1) + (a * a) * self._fn(4, x)
+
+    def int_H(self, x, a):
+        """∫_0^x H dx, Smith+2018 eq. 13 (src/ConstantsAndFunctions.jl:246-256)."""
+        return K.Sqrt_pi_over_2 * self._fn(0, x) - K.inv_Sqrt_pi_over_2 * a * self._fn(1, x) + (a * a) * self._fn(3, x)
+
+    def htophat(self, x, xmax, xmin, a):
+        """Htophat (src/ConstantsAndFunctions.jl:265-279)."""
+        dx = xmax - xmin
+        with np.errstate(invalid="ignore", divide="ignore"):
+            avg = (self.int_H(xmax, a) - self.int_H(xmin, a)) / dx
+        return np.where(dx > 1e-5, avg, self.H(x, a))
+
+
+def update_hvoigt(lut: VoigtLUT, dnu_rel: np.ndarray, dnuC_rel: np.ndarray, a: np.ndarray,
+                  inv_dnuD: np.ndarray) -> np.ndarray:
+    """Update_Hvoigt! (src/ConstantsAndFunctions.jl:319-357).  a, inv_dnuD: (N_Z, N_r).
+    Returns Hvoigt (N_Z, N_r, N_ν).  Cells sharing (a, ΔνD) share a column."""
+    a2 = a.reshape(-1)
+    ic = inv_dnuD.reshape(-1)
+    keys = np.stack([a2, ic], axis=1)
+    uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+    cols = np.empty((len(uniq), len(dnuC_rel)))
+    for u, (au, iu) in enumerate(uniq):
+        x = dnuC_rel * iu
+        xmx = dnu_rel[1:] * iu
+        xmn = dnu_rel[:-1] * iu
+        cols[u] = lut.htophat(x, xmx, xmn, au)
+    return cols[inv.reshape(-1)].reshape(a.shape + (len(dnuC_rel),))
+
+
+def update_phi(dnu: np.ndarray, Hvoigt: np.ndarray, kmin: int, kmax: int) -> np.ndarray:
+    """Update_ϕν! over the core bins kmin..kmax (0-based, inclusive)
+    (src/ConstantsAndFunctions.jl:362-393).  Sums run sequentially in k."""
+    phi = np.zeros_like(Hvoigt)
+    flat = Hvoigt.reshape(-1, Hvoigt.shape[-1])
+    out = phi.reshape(-1, Hvoigt.shape[-1])
+    s = np.zeros(flat.shape[0])
+    for k in range(kmin, kmax + 1):
+        s = s + dnu[k] * flat[:, k]
+    inv = 1.0 / s
+    out[:, kmin:kmax + 1] = flat[:, kmin:kmax + 1] * inv[:, None]
+    return phi
+
+
+@dataclass
+class LyaArrays:
+    """Host mirror of LyαArrays{FluxMethod} (src/Structs.jl:23-124).
+
+    Spectral arrays have numpy shape (N_Z, N_r, N_ν) (== Julia [k,i,j]); cell
+    arrays (N_Z, N_r).  The authoritative state lives on the device once a
+    context is attached (see lydion_amd.m1lya)."""
+    nu: np.ndarray
+    c_lya: float
+    n_r: int
+    n_z: int
+    recoil: bool = True
+    hll: bool = False
+    kcore: tuple = (0, 0)
+    alpha_floor: float = 0.0
+    arrays: dict = field(default_factory=dict)
+
+    @property
+    def n_nu(self) -> int:
+        return len(self.nu) - 1
+
+    @property
+    def dnu(self):
+        return np.diff(self.nu)
+
+    @property
+    def nuC(self):
+        return self.nu[:-1] + 0.5 * self.dnu
+
+    def __getitem__(self, name):
+        return self.arrays[name]
+
+    def __setitem__(self, name, value):
+        self.arrays[name] = value
+
+
+def generate_lya_arrays(c_lya: float, r: np.ndarray, Z: np.ndarray, T: np.ndarray, nu: np.ndarray,
+                        recoil: bool = True, hll: bool = False, lut: VoigtLUT | None = None) -> LyaArrays:
+    """GenerateLyαArrays_FromSaved (src/SetupLymanAlphaArrays.jl:244-393) for a given ν grid:
+    zeroed state, Voigt profile Hvoigt, core emission profile ϕ_ν, opacity floor."""
+    n_z, n_r = T.shape
+    n_nu = len(nu) - 1
+    dnu = np.diff(nu)
+    nuC = nu[:-1] + 0.5 * dnu
+    assert np.all(dnu > 0)
+    L = LyaArrays(nu=np.asarray(nu, dtype=np.float64), c_lya=c_lya, n_r=n_r, n_z=n_z, recoil=recoil, hll=hll)
+    L.kcore = core_range(nuC)
+    L.alpha_floor = K.tau_alpha_floor * 1 / max(np.max(r), np.max(Z))
+    L["nuC"] = nuC
+    L["dnu"] = dnu
+    L["dnu_rel"] = nu - K.nu_Lya
+    L["dnuC_rel"] = nuC - K.nu_Lya
+    L["inv_dnu"] = 1 / dnu
+    L["inv_nuC"] = 1 / nuC
+    a = compute_a(T)
+    dnuD = compute_dnuD(T)
+    L["a"], L["dnuD"], L["inv_dnuD"] = a, dnuD, 1.0 / dnuD
+    lut = lut or VoigtLUT()
+    L["Hvoigt"] = update_hvoigt(lut, L["dnu_rel"], L["dnuC_rel"], a, L["inv_dnuD"])
+    L["phi"] = update_phi(dnu, L["Hvoigt"], *L.kcore)
+    shape3 = (n_z, n_r, n_nu)
+    for name in ("J", "J_old", "H_r", "H_z", "H_r_old", "H_z_old", "alpha_F"):
+        L[name] = np.zeros(shape3)
+    for name in ("alpha", "alpha_c", "pd", "js", "drhou_r", "drhou_z", "f_r", "f_z", "e_lya"):
+        L[name] = np.zeros((n_z, n_r))
+    return L
