@@ -230,7 +230,8 @@ int m1lya_kernel_stats(m1lya_ctx *ctx, int index, char *name, size_t name_len,
 int m1lya_reset_kernel_stats(m1lya_ctx *ctx);
 
 /* Runs n IEEE operations on the device for the bit-exactness self-test:
- * op 0: a/b, 1: sqrt(a), 2: the library's exp(a). */
+ * op 0: a/b, 1: sqrt(a), 2: the library's exp(a), 3: a plain copy out = a
+ * (one kernel reading and writing n doubles; calibrates HBM byte counters). */
 int m1lya_selftest_math(int op, const double *a, const double *b, double *out, size_t n);
 
 #ifdef __cplusplus

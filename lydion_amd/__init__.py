@@ -13,13 +13,13 @@ from .lya_arrays import LyaArrays, VoigtLUT, bins_freq_grid, generate_lya_arrays
 from .m1lya import (M1LyaContext, advect_lya, compute_f_esc_lya_cylindrical, edge_spectra, find_active_bins,
                     frequency_update_j_lya_all_cells, lya_emission_absorption_update, reset_drhou, run_steps,
                     step, update_e_lya, update_f_lya_mf_l_lya, update_freq_factors)
-from .static_cloud import StaticCloud, config1, config2, point_source_cloud, setup_static_cloud
+from .static_cloud import StaticCloud, static_cloud_32x32x64, cone_dust_cloud_256x256x128, point_source_cloud, setup_static_cloud
 
 __all__ = [
     "constants", "GLF", "HLL", "OUTFLOW", "REFLECT", "M1LyaError", "GeometryCylindrical", "setup_cylindrical_grid",
     "LyaArrays", "VoigtLUT", "bins_freq_grid", "generate_lya_arrays", "reference_freq_grid", "M1LyaContext",
     "advect_lya", "compute_f_esc_lya_cylindrical", "edge_spectra", "find_active_bins",
     "frequency_update_j_lya_all_cells", "lya_emission_absorption_update", "reset_drhou", "run_steps", "step",
-    "update_e_lya", "update_f_lya_mf_l_lya", "update_freq_factors", "StaticCloud", "config1", "config2",
+    "update_e_lya", "update_f_lya_mf_l_lya", "update_freq_factors", "StaticCloud", "static_cloud_32x32x64", "cone_dust_cloud_256x256x128",
     "point_source_cloud", "setup_static_cloud",
 ]

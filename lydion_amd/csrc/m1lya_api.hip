@@ -899,7 +899,7 @@ int m1lya_reset_kernel_stats(m1lya_ctx* c)
 
 int m1lya_selftest_math(int op, const double* a, const double* b, double* out, size_t n)
 {
-    if (!a || !out || (op == 0 && !b) || op < 0 || op > 2) {
+    if (!a || !out || (op == 0 && !b) || op < 0 || op > 3) {
         g_last_error = "selftest: bad arguments";
         return M1LYA_ERR_ARG;
     }

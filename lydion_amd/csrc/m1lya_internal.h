@@ -10,6 +10,13 @@ namespace m1 {
 // Cell columns per wavefront in the column kernel (lane-per-column dgtsv).
 constexpr int kColumnW = 4;
 
+// LDS row stride of one column's tridiagonal arrays: >= N_ν - 2 rows, odd.
+__host__ __device__ inline int col_stride(int nn)
+{
+    int n = nn - 2 > 1 ? nn - 2 : 1;
+    return n | 1;
+}
+
 struct DevGeom {
     int nr, nz;
     const double* A_r;        // [nz][nr+1]

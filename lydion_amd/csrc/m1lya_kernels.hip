@@ -264,69 +264,60 @@ __global__ __launch_bounds__(256) void k_advect(AdvectArgs a)
 
 // ------------------------------------------------------------------------
 // LAPACK dgtsv (NRHS = 1, published reference algorithm: elimination with
-// partial pivoting by row interchange) on one LDS-resident column.
-// Called by TridiagonalSolve, src/TridiagonalSolver.jl:54.
+// partial pivoting by row interchange) on one LDS-resident column, as called by
+// TridiagonalSolve (src/TridiagonalSolver.jl:54).  Row i of each array is at
+// [i]; the solution overwrites b.  The current and next rows are carried in
+// registers, so the dependent chain is arithmetic only; every value stored
+// is the one LAPACK would store, computed by the same operations.
 // ------------------------------------------------------------------------
-__device__ int gtsv_col(int n, double* dl, double* d, double* du, double* b, int st)
+__device__ int gtsv_col(int n, double* __restrict__ dl, double* __restrict__ d, double* __restrict__ du,
+                        double* __restrict__ b)
 {
-    // element m of an array lives at [m * st]  (st = W: columns interleaved)
-#define DL(m) dl[(m) * st]
-#define D(m) d[(m) * st]
-#define DU(m) du[(m) * st]
-#define B(m) b[(m) * st]
-    for (int i = 0; i < n - 2; ++i) {
-        if (fabs(D(i)) >= fabs(DL(i))) {
-            if (D(i) != 0.0) {
-                double fact = DL(i) / D(i);
-                D(i + 1) = D(i + 1) - fact * DU(i);
-                B(i + 1) = B(i + 1) - fact * B(i);
-            } else {
-                return i + 1;
-            }
-            DL(i) = 0.0;
+    double d_i = d[0], b_i = b[0];
+    double du_i = (n > 1) ? du[0] : 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+        const bool last = (i == n - 2);
+        const double dl_i = dl[i];
+        double d_n = d[i + 1], b_n = b[i + 1];
+        double du_n = last ? 0.0 : du[i + 1];
+        if (fabs(d_i) >= fabs(dl_i)) {
+            if (d_i == 0.0) return i + 1;
+            double fact = dl_i / d_i;
+            d_n = d_n - fact * du_i;
+            b_n = b_n - fact * b_i;
+            if (!last) dl[i] = 0.0;
         } else {
-            double fact = D(i) / DL(i);
-            D(i) = DL(i);
-            double temp = D(i + 1);
-            D(i + 1) = DU(i) - fact * temp;
-            DL(i) = DU(i + 1);
-            DU(i + 1) = -fact * DL(i);
-            DU(i) = temp;
-            temp = B(i);
-            B(i) = B(i + 1);
-            B(i + 1) = temp - fact * B(i + 1);
-        }
-    }
-    if (n > 1) {
-        int i = n - 2;
-        if (fabs(D(i)) >= fabs(DL(i))) {
-            if (D(i) != 0.0) {
-                double fact = DL(i) / D(i);
-                D(i + 1) = D(i + 1) - fact * DU(i);
-                B(i + 1) = B(i + 1) - fact * B(i);
-            } else {
-                return i + 1;
+            double fact = d_i / dl_i;
+            double new_d_i = dl_i;
+            double temp = d_n;
+            d_n = du_i - fact * temp;
+            if (!last) {
+                dl[i] = du_n;             // fill-in of the second super-diagonal
+                du_n = -fact * du_n;
             }
-        } else {
-            double fact = D(i) / DL(i);
-            D(i) = DL(i);
-            double temp = D(i + 1);
-            D(i + 1) = DU(i) - fact * temp;
-            DU(i) = temp;
-            temp = B(i);
-            B(i) = B(i + 1);
-            B(i + 1) = temp - fact * B(i + 1);
+            du_i = temp;
+            double tb = b_i;
+            b_i = b_n;
+            b_n = tb - fact * b_n;
+            d_i = new_d_i;
         }
+        d[i] = d_i; du[i] = du_i; b[i] = b_i;
+        d_i = d_n; b_i = b_n; du_i = du_n;
     }
-    if (D(n - 1) == 0.0) return n;
-    B(n - 1) = B(n - 1) / D(n - 1);
-    if (n > 1) B(n - 2) = (B(n - 2) - DU(n - 2) * B(n - 1)) / D(n - 2);
-    for (int i = n - 3; i >= 0; --i) B(i) = (B(i) - DU(i) * B(i + 1) - DL(i) * B(i + 2)) / D(i);
+    if (d_i == 0.0) return n;
+    d[n - 1] = d_i;
+    double x1 = b_i / d_i;
+    b[n - 1] = x1;
+    if (n == 1) return 0;
+    double x0 = (b[n - 2] - du[n - 2] * x1) / d[n - 2];
+    b[n - 2] = x0;
+    for (int i = n - 3; i >= 0; --i) {
+        double x = (b[i] - du[i] * x0 - dl[i] * x1) / d[i];
+        b[i] = x;
+        x1 = x0;
+        x0 = x;
+    }
     return 0;
-#undef DL
-#undef D
-#undef DU
-#undef B
 }
 
 // ------------------------------------------------------------------------
@@ -350,11 +341,13 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
     const int flags = a.flags;
     const bool EMIT = flags & COL_EMIT, FREQ = flags & COL_FREQ, STOREF = flags & COL_STORE_F;
     const bool SOLVE = flags & COL_SOLVE, HCORR = flags & COL_HCORR, FUSED = flags & COL_FUSED;
-    const size_t rows = (size_t)(n > 0 ? n : 1);
+    // [column][row] with an odd row stride: conflict-free for both the lanes-over-k
+    // writes (consecutive rows) and the lane-per-column sweep (stride S, S odd).
+    const int S = col_stride(nn);
     double* sDL = lds;
-    double* sD = lds + rows * W;
-    double* sDU = lds + 2 * rows * W;
-    double* sB = lds + 3 * rows * W;
+    double* sD = lds + (size_t)S * W;
+    double* sDU = lds + (size_t)2 * S * W;
+    double* sB = lds + (size_t)3 * S * W;
     const int ncells = nr * nz;
     const int ngroups = (ncells + W - 1) / W;
     const double ct = a.c_tilde, dt = a.dt, dw = a.dw;
@@ -495,10 +488,10 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                         // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215) in gtsv form:
                         // dl <- A[2:end], du <- C[1:end-1] (src/TridiagonalSolver.jl:45-46)
                         const int row = k - 1;
-                        sD[(size_t)row * W + w] = -(Fk + dw);
-                        sB[(size_t)row * W + w] = -dw * Jo[w][kk];
-                        if (row >= 1) sDL[(size_t)(row - 1) * W + w] = Fkm;
-                        if (row <= n - 2) sDU[(size_t)row * W + w] = Fkp;
+                        sD[(size_t)w * S + row] = -(Fk + dw);
+                        sB[(size_t)w * S + row] = -dw * Jo[w][kk];
+                        if (row >= 1) sDL[(size_t)w * S + row - 1] = Fkm;
+                        if (row <= n - 2) sDU[(size_t)w * S + row] = Fkp;
                     }
                 }
             }
@@ -508,7 +501,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
             __syncthreads();
             const int c = grp * W + lane;
             if (lane < W && c < ncells && n > 0) {
-                int info = gtsv_col(n, sDL + lane, sD + lane, sDU + lane, sB + lane, W);
+                int info = gtsv_col(n, sDL + (size_t)lane * S, sD + (size_t)lane * S, sDU + (size_t)lane * S,
+                                    sB + (size_t)lane * S);
                 if (info != 0) atomicMax(a.err_flag, 1);
             }
             __syncthreads();
@@ -531,7 +525,7 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                 double jn = Jo[w][kk];
                 double hr = Hr[w][kk], hz = Hz[w][kk];
                 if (SOLVE) {
-                    if (k >= 1 && k <= nn - 2) jn = sB[(size_t)(k - 1) * W + w];
+                    if (k >= 1 && k <= nn - 2) jn = sB[(size_t)w * S + (k - 1)];
                     else if (!FUSED) jn = a.J[idx];
                 }
                 if (HCORR) {
@@ -742,7 +736,8 @@ __global__ void k_selftest(int op, const double* a, const double* b, double* out
     if (t >= n) return;
     if (op == 0) out[t] = a[t] / b[t];
     else if (op == 1) out[t] = sqrt(a[t]);
-    else out[t] = exp_det(a[t]);
+    else if (op == 2) out[t] = exp_det(a[t]);
+    else out[t] = a[t];
 }
 
 // ------------------------------------------------------------------------
@@ -775,8 +770,7 @@ void launch_advect(hipStream_t s, const AdvectArgs& a)
 template <int W, int KPL>
 static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
 {
-    size_t rows = (size_t)(a.nn - 2 > 0 ? a.nn - 2 : 1);
-    size_t lds = 4 * rows * W * sizeof(double);
+    size_t lds = 4 * (size_t)col_stride(a.nn) * W * sizeof(double);
     hipLaunchKernelGGL((k_column<W, KPL>), dim3(nblocks), dim3(64), lds, s, a);
 }
 

@@ -1,0 +1,210 @@
+"""
+    M1LyaGPU
+
+Drop-in GPU backend for LYDION's Lyα M1 step.  Defines methods with the names
+and signatures of the reference functions (src/UpdateLyaJandH.jl,
+src/ConstantsAndFunctions.jl, src/FreqFactors.jl) whose bodies `ccall` into
+libm1lya.so (include/m1lya.h).  The state lives on the device, keyed by the
+LyαArrays object; `download_state!` copies it back at save/checkpoint points.
+
+Usage in src/StaticCloudTest.jl (after GenerateLyαArrays / UpdateFreqFactors!):
+
+    using .M1LyaGPU
+    ctx = M1LyaGPU.attach!(LyaState, geometry, T, u_r_star, u_Z_star,
+                           ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r)
+    # the time loop body is unchanged: the methods below shadow the CPU ones
+"""
+module M1LyaGPU
+
+export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
+       FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
+       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!
+
+const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
+
+# enum m1lya_field (include/m1lya.h)
+const F_J, F_J_OLD, F_H_R, F_H_Z, F_H_R_OLD, F_H_Z_OLD = 0, 1, 2, 3, 4, 5
+const F_ALPHA_F, F_HVOIGT, F_PHI_NU, F_FIJ_KP, F_FIJ_K, F_FIJ_KM = 6, 7, 8, 9, 10, 11
+const F_ALPHA_LYA, F_ALPHA_C, F_PD, F_JS, F_T, F_DNU_D = 12, 13, 14, 15, 16, 17
+const F_DUR_DR, F_DUR_DZ, F_DUZ_DR, F_DUZ_DZ, F_UR_OVER_R = 18, 19, 20, 21, 22
+const F_DRHOU_R, F_DRHOU_Z, F_F_LYA_R, F_F_LYA_Z, F_E_LYA = 23, 24, 25, 26, 27
+const F_UR_STAR, F_UZ_STAR = 28, 29
+
+bc(s::Symbol) = s === :Reflect ? Cint(0) : s === :Outflow ? Cint(1) : error("unknown BC $s")
+
+struct CGeometry           # m1lya_geometry
+    r::Ptr{Float64}; Z::Ptr{Float64}; rCoM::Ptr{Float64}; ZCoM::Ptr{Float64}
+    dV::Ptr{Float64}; inv_dV::Ptr{Float64}; A_r::Ptr{Float64}; A_Z::Ptr{Float64}; inv_r_cell::Ptr{Float64}
+    dr_L::Ptr{Float64}; dr_R::Ptr{Float64}; inv_dr_i_im1::Ptr{Float64}; inv_dr_ip1_i::Ptr{Float64}
+    inv_dr_ip2_ip1::Ptr{Float64}; dr_inner::NTuple{5,Float64}; dr_outer::NTuple{5,Float64}
+    dZ_L::Ptr{Float64}; dZ_R::Ptr{Float64}; inv_dZ_j_jm1::Ptr{Float64}; inv_dZ_jp1_j::Ptr{Float64}
+    inv_dZ_jp2_jp1::Ptr{Float64}; dZ_inner::NTuple{5,Float64}; dZ_outer::NTuple{5,Float64}
+end
+
+struct CFreqGrid           # m1lya_freq_grid
+    nu::Ptr{Float64}; nuC::Ptr{Float64}; dnu::Ptr{Float64}; inv_dnu::Ptr{Float64}; inv_nuC::Ptr{Float64}
+    kcore_min::Cint; kcore_max::Cint; recoil::Cint; flux_method::Cint; alpha_floor::Float64
+end
+
+const CONTEXTS = IdDict{Any,Ptr{Cvoid}}()
+
+function check(rc::Cint, ctx::Ptr{Cvoid})
+    rc == 0 && return nothing
+    buf = Vector{UInt8}(undef, 1024)
+    ccall((:m1lya_last_error, libm1lya), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Csize_t), ctx, buf, length(buf))
+    error("libm1lya error $rc: " * unsafe_string(pointer(buf)))
+end
+
+ctxof(Lyα) = get(CONTEXTS, Lyα) do
+    error("M1LyaGPU.attach! has not been called for this LyαArrays")
+end
+
+upload!(ctx, field, A::AbstractArray{Float64}) = GC.@preserve A check(
+    ccall((:m1lya_upload, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+          ctx, field, A, length(A)), ctx)
+download!(ctx, field, A::AbstractArray{Float64}) = GC.@preserve A check(
+    ccall((:m1lya_download, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+          ctx, field, A, length(A)), ctx)
+optr(A) = A === nothing ? Ptr{Float64}(C_NULL) : pointer(A)
+
+"""Creates the device context of a LyαArrays + GeometryCylindrical and uploads
+the static coefficients and the state (zero-copy: Julia arrays are [k,i,j])."""
+function attach!(Lyα, g, T, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r)
+    N_ν, N_r, N_Z = size(Lyα.J_Lyα)
+    ref = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:m1lya_create, libm1lya), Cint, (Cint, Cint, Cint, Cint, Ref{Ptr{Cvoid}}),
+                N_ν, N_r, N_Z, 0, ref), C_NULL)
+    ctx = ref[]
+    arrays = (g.r, g.Z, g.rCoM, g.ZCoM, g.ΔV, g.inv_ΔV, g.A_r, g.A_Z, g.inv_r_cell,
+              g.Δr_L, g.Δr_R, g.inv_dr_i_im1, g.inv_dr_ip1_i, g.inv_dr_ip2_ip1,
+              g.ΔZ_L, g.ΔZ_R, g.inv_dZ_j_jm1, g.inv_dZ_jp1_j, g.inv_dZ_jp2_jp1)
+    GC.@preserve arrays begin
+        cg = CGeometry(pointer(g.r), pointer(g.Z), pointer(g.rCoM), pointer(g.ZCoM), pointer(g.ΔV),
+                       pointer(g.inv_ΔV), pointer(g.A_r), pointer(g.A_Z), pointer(g.inv_r_cell),
+                       pointer(g.Δr_L), pointer(g.Δr_R), pointer(g.inv_dr_i_im1), pointer(g.inv_dr_ip1_i),
+                       pointer(g.inv_dr_ip2_ip1),
+                       (g.Δr_L_InnerBoundary, g.Δr_R_InnerBoundary, g.inv_dr_i_im1_InnerBoundary,
+                        g.inv_dr_ip1_i_InnerBoundary, g.inv_dr_ip2_ip1_InnerBoundary),
+                       (g.Δr_L_OuterBoundary, g.Δr_R_OuterBoundary, g.inv_dr_i_im1_OuterBoundary,
+                        g.inv_dr_ip1_i_OuterBoundary, g.inv_dr_ip2_ip1_OuterBoundary),
+                       pointer(g.ΔZ_L), pointer(g.ΔZ_R), pointer(g.inv_dZ_j_jm1), pointer(g.inv_dZ_jp1_j),
+                       pointer(g.inv_dZ_jp2_jp1),
+                       (g.ΔZ_L_InnerBoundary, g.ΔZ_R_InnerBoundary, g.inv_dZ_j_jm1_InnerBoundary,
+                        g.inv_dZ_jp1_j_InnerBoundary, g.inv_dZ_jp2_jp1_InnerBoundary),
+                       (g.ΔZ_L_OuterBoundary, g.ΔZ_R_OuterBoundary, g.inv_dZ_j_jm1_OuterBoundary,
+                        g.inv_dZ_jp1_j_OuterBoundary, g.inv_dZ_jp2_jp1_OuterBoundary))
+        check(ccall((:m1lya_set_geometry, libm1lya), Cint, (Ptr{Cvoid}, Ref{CGeometry}), ctx, cg), ctx)
+    end
+    fm = Lyα.FluxMethod isa Val{:HLL} ? Cint(1) : Cint(0)
+    GC.@preserve Lyα begin
+        fg = CFreqGrid(pointer(Lyα.ν), pointer(Lyα.νC), pointer(Lyα.Δν), pointer(Lyα.inv_Δν),
+                       pointer(Lyα.inv_νC), Lyα.kcore_min - 1, Lyα.kcore_max - 1, Lyα.Recoil, fm,
+                       Lyα.α_Lyα_floor)
+        check(ccall((:m1lya_set_freq_grid, libm1lya), Cint, (Ptr{Cvoid}, Ref{CFreqGrid}), ctx, fg), ctx)
+    end
+    for (f, A) in ((F_J, Lyα.J_Lyα), (F_J_OLD, Lyα.J_Lyα_old), (F_H_R, Lyα.H_r_Lyα), (F_H_Z, Lyα.H_Z_Lyα),
+                   (F_H_R_OLD, Lyα.H_r_Lyα_old), (F_H_Z_OLD, Lyα.H_Z_Lyα_old), (F_ALPHA_F, Lyα.α_F_Lyα),
+                   (F_HVOIGT, Lyα.Hvoigt), (F_PHI_NU, Lyα.ϕ_ν), (F_ALPHA_LYA, Lyα.α_Lyα), (F_ALPHA_C, Lyα.αc_Lyα),
+                   (F_PD, Lyα.pd), (F_JS, Lyα.∫dν_js_Lyα), (F_DNU_D, Lyα.ΔνD), (F_T, T),
+                   (F_DRHOU_R, Lyα.Δρu_Lyα_r), (F_DRHOU_Z, Lyα.Δρu_Lyα_Z), (F_UR_STAR, u_r_star),
+                   (F_UZ_STAR, u_Z_star), (F_DUR_DR, ∂u_r_∂r), (F_DUR_DZ, ∂u_r_∂Z), (F_DUZ_DR, ∂u_Z_∂r),
+                   (F_DUZ_DZ, ∂u_Z_∂Z), (F_UR_OVER_R, u_r_over_r))
+        upload!(ctx, f, A)
+    end
+    CONTEXTS[Lyα] = ctx
+    return ctx
+end
+
+"""Copies the device state back into the LyαArrays (SaveTimeStepHDF5, /Checkpoint)."""
+function download_state!(Lyα)
+    ctx = ctxof(Lyα)
+    for (f, A) in ((F_J, Lyα.J_Lyα), (F_J_OLD, Lyα.J_Lyα_old), (F_H_R, Lyα.H_r_Lyα), (F_H_Z, Lyα.H_Z_Lyα),
+                   (F_H_R_OLD, Lyα.H_r_Lyα_old), (F_H_Z_OLD, Lyα.H_Z_Lyα_old), (F_F_LYA_R, Lyα.f_Lyα_r),
+                   (F_F_LYA_Z, Lyα.f_Lyα_Z), (F_DRHOU_R, Lyα.Δρu_Lyα_r), (F_DRHOU_Z, Lyα.Δρu_Lyα_Z))
+        download!(ctx, f, A)
+    end
+    return Lyα
+end
+
+# --- src/UpdateLyaJandH.jl:107-124 ---
+function AdvectLyα!(c̃Δt::Float64, Lyα, geometry, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r,
+                    ∂u_Z_∂Z, u_r_over_r, BC_r_left::Symbol=:Reflect, BC_r_right::Symbol=:Outflow,
+                    BC_Z_bot::Symbol=:Reflect, BC_Z_top::Symbol=:Outflow)
+    ctx = ctxof(Lyα)
+    GC.@preserve u_r_star u_Z_star ∂u_r_∂r ∂u_r_∂Z ∂u_Z_∂r ∂u_Z_∂Z u_r_over_r check(
+        ccall((:m1lya_advect_lya, libm1lya), Cint,
+              (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+               Ptr{Float64}, Ptr{Float64}, Cint, Cint, Cint, Cint),
+              ctx, c̃Δt, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r,
+              bc(BC_r_left), bc(BC_r_right), bc(BC_Z_bot), bc(BC_Z_top)), ctx)
+    return nothing
+end
+
+# --- src/ConstantsAndFunctions.jl:483-500 (arrays are the LyαArrays members, already resident) ---
+function LyαEmissionAbsorptionUpdate!(J_Lyα, J_Lyα_old, H_r_Lyα, H_Z_Lyα, H_r_old_Lyα, H_Z_old_Lyα,
+                                      ∫dν_js_Lyα, ϕ_ν, Hvoigt, α_Lyα, αc_Lyα, α_F_Lyα, pd, Δρu_Lyα_r,
+                                      Δρu_Lyα_Z, Δν, c̃::Float64, Δt::Float64)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.J_Lyα === J_Lyα))
+    check(ccall((:m1lya_emission_absorption_update, libm1lya), Cint, (Ptr{Cvoid}, Float64, Float64),
+                ctx, c̃, Δt), ctx)
+    return nothing
+end
+
+# --- src/UpdateLyaJandH.jl:231-244 ---
+function FrequencyUpdateJLyα_AllCells!(c̃Δt::Float64, Lyα, T, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z,
+                                       u_r_over_r, UpdateFreqFactors::Bool=true)
+    ctx = ctxof(Lyα)
+    GC.@preserve T ∂u_r_∂r ∂u_r_∂Z ∂u_Z_∂r ∂u_Z_∂Z u_r_over_r check(
+        ccall((:m1lya_frequency_update_all_cells, libm1lya), Cint,
+              (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+               Ptr{Float64}, Cint),
+              ctx, c̃Δt, T, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r, UpdateFreqFactors), ctx)
+    return nothing
+end
+
+# --- src/FreqFactors.jl:142-149 ---
+function UpdateFreqFactors!(Lyα, T, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r)
+    ctx = ctxof(Lyα)
+    GC.@preserve T ∂u_r_∂r ∂u_r_∂Z ∂u_Z_∂r ∂u_Z_∂Z u_r_over_r check(
+        ccall((:m1lya_update_freq_factors, libm1lya), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+              ctx, T, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r), ctx)
+    return nothing
+end
+
+# --- src/ConstantsAndFunctions.jl:588-599 (f_Lyα_r/Z are refreshed on the device) ---
+function Update_f_Lyα_MF_L_Lyα!(f_Lyα_r, f_Lyα_Z, Δρu_Lyα_r, Δρu_Lyα_Z, ∫dν_js, ΔV, rCoM, ZCoM,
+                                Δt::Float64, Z_center::Float64, BC_Z_bot::Symbol=:Reflect)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.f_Lyα_r === f_Lyα_r))
+    out = zeros(4)
+    check(ccall((:m1lya_update_f_mf_l, libm1lya), Cint, (Ptr{Cvoid}, Float64, Float64, Cint, Ptr{Float64}),
+                ctx, Δt, Z_center, bc(BC_Z_bot), out), ctx)
+    return out[1], out[2], out[3], out[4]
+end
+
+# --- src/ConstantsAndFunctions.jl:969-976 ---
+function Compute_f_esc_Lyα_Cylindrical(H_r_Lya, H_Z_Lya, A_r, A_Z, Δν, L_Lyα_allspace::Float64,
+                                       BC_Z_bot::Symbol=:Reflect)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.H_r_Lyα === H_r_Lya))
+    f = Ref{Float64}(0.0)
+    check(ccall((:m1lya_compute_f_esc, libm1lya), Cint, (Ptr{Cvoid}, Float64, Cint, Ref{Float64}),
+                ctx, L_Lyα_allspace, bc(BC_Z_bot), f), ctx)
+    return f[]
+end
+
+# --- src/ConstantsAndFunctions.jl:809-816 ---
+function Reset_Δρu!(Δρu_r, Δρu_Z)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.Δρu_Lyα_r === Δρu_r))
+    check(ccall((:m1lya_reset_drhou, libm1lya), Cint, (Ptr{Cvoid},), ctx), ctx)
+    return nothing
+end
+
+# --- src/ConstantsAndFunctions.jl:416-420 ---
+function Update_e_Lyα!(e_Lyα, Δν, J)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.J_Lyα === J))
+    check(ccall((:m1lya_update_e_lya, libm1lya), Cint, (Ptr{Cvoid},), ctx), ctx)
+    download!(ctx, F_E_LYA, e_Lyα)
+    return nothing
+end
+
+end # module

@@ -160,12 +160,12 @@ def point_source_cloud(n_r: int, n_z: int, n_nu: int, tau0: float, T0: float = 1
                               bc_z_bot=bc_z_bot, n_r=n_r, n_z=n_z, nu=config_grid(n_nu, tau0), lut=lut)
 
 
-def config1(lut: VoigtLUT | None = None) -> StaticCloud:
+def static_cloud_32x32x64(lut: VoigtLUT | None = None) -> StaticCloud:
     """32×32 r-Z, 64 bins, τ0 = 1e6, T = 1e4 K, point source, no dust, no flow."""
     return point_source_cloud(32, 32, 64, 1e6, lut=lut)
 
 
-def config2(lut: VoigtLUT | None = None, n_r: int = 256, n_z: int = 256, n_nu: int = 128) -> StaticCloud:
+def cone_dust_cloud_256x256x128(lut: VoigtLUT | None = None, n_r: int = 256, n_z: int = 256, n_nu: int = 128) -> StaticCloud:
     """256×256 r-Z, 128 bins, τ0 = 1e8, T = 1e4 K, polar cone of half-angle π/4 with
     τcone = 1e-6 τ0 (smooth, Δμ = 0.025), τ_abs = 1, point source of radius 0.5 cells."""
     return point_source_cloud(n_r, n_z, n_nu, 1e8, tau_abs=1.0, theta_cone=math.pi / 4, lut=lut)
@@ -186,3 +186,30 @@ def random_state(shape: tuple, seed: int = DEFAULT_SEED, scale: float = 1.0, jep
     Hr = f * J * np.cos(theta)
     Hz = f * J * np.sin(theta)
     return J, Hr, Hz
+
+
+def developed_state(cloud: StaticCloud, seed: int = DEFAULT_SEED, floor: float = 1e-6, noise: float = 0.05):
+    """A seeded, smooth 'developed' radiation field for benchmarking (not a solution of the
+    equations): the static-sphere emergent-spectrum shape (x²/aτ) sech²(√(π³/54)|x|³/aτ)
+    (src/ConstantsAndFunctions.jl:1018-1025) at every cell, scaled by 1/(1 + (R/Δr)²), with a
+    relative floor so that every frequency bin is active, and a radial flux H = f J R̂ with
+    f growing from 0.05 at the centre to 0.6 at the box edge."""
+    lya, g = cloud.lya, cloud.geometry
+    rng = np.random.Generator(np.random.PCG64(seed))
+    T0 = cloud.params["T0"]
+    a_tau = compute_a(T0) * cloud.params["tau_cl"]
+    x = (lya["nuC"] - K.nu_Lya) / float(np.asarray(lya["dnuD"]).reshape(-1)[0])
+    prof = (x * x / a_tau) / np.cosh(np.sqrt(np.pi ** 3 / 54.0) * np.abs(x) ** 3 / a_tau) ** 2
+    prof = prof / prof.max() + floor
+    rc = g.rCoM[None, :]
+    zc = g.ZCoM[:, None] - cloud.Zc
+    R = np.sqrt(rc * rc + zc * zc)
+    dr = g.r[1] - g.r[0]
+    amp = 1e-3 / (1.0 + (R / dr) ** 2)
+    J = amp[:, :, None] * prof[None, None, :] * np.exp(noise * rng.normal(size=(lya.n_z, lya.n_r, lya.n_nu)))
+    f = 0.05 + 0.55 * (R / R.max())
+    Hr = (f * rc / R)[:, :, None] * J
+    Hz = (f * zc / R)[:, :, None] * J
+    for k, v in (("J", J), ("H_r", Hr), ("H_z", Hz), ("J_old", J), ("H_r_old", Hr), ("H_z_old", Hz)):
+        lya[k] = np.ascontiguousarray(v)
+    return cloud

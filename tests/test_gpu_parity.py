@@ -118,10 +118,10 @@ def test_reductions_match(lut):
     assert rel(ng, no) < RED_TOL
 
 
-@pytest.mark.parametrize("case", ["config1", "cone_dust_flow"])
+@pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow"])
 def test_fused_step_bit_exact(lut, case):
-    if case == "config1":
-        cloud = L.config1(lut=lut)
+    if case == "static_cloud_32x32x64":
+        cloud = L.static_cloud_32x32x64(lut=lut)
         nsteps = 40
     else:
         cloud = randomize(small_cloud(n_r=24, n_z=20, n_nu=128, tau0=1e8, tau_abs=1.0, theta_cone=np.pi / 4,
@@ -143,7 +143,7 @@ def test_fused_step_bit_exact(lut, case):
 
 
 def test_run_steps_matches_single_steps(lut):
-    cloud = L.config1(lut=lut)
+    cloud = L.static_cloud_32x32x64(lut=lut)
     c1 = M.M1LyaContext(cloud.lya, cloud.geometry)
     c2 = M.M1LyaContext(cloud.lya, cloud.geometry)
     hist = M.run_steps(c1, 12, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot)
