@@ -219,6 +219,64 @@ int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
 int m1lya_run_steps(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
                     int bc_r_right, int bc_z_bot, int bc_z_top, double *history);
 
+/* ---- mixed-precision batched step ---------------------------------------
+ * A batch advances many clouds of uneven size (N_r, N_Z >= 3, 3 <= N_ν <= 128),
+ * optical depth and frequency resolution by the same fused step as m1lya_step
+ * (the RT-only loop body of src/StaticCloudTest.jl:357-392), one launch per step
+ * for the whole batch.  Precision: J, H_r, H_Z, α_F, Hvoigt and ϕ_ν are stored
+ * in FP32 (J and H divided by a per-cloud scale J0); the flux sweep and its M1
+ * closure run in FP32, with the cell-centred closure in FP64 near free
+ * streaming (f > 0.9); emission/absorption, the frequency factors and the
+ * tridiagonal frequency solve run in FP64; M_F, L, f_esc are accumulated with
+ * compensated FP64 sums.  A batch of one cloud is the single-cloud FP32 step.
+ * Differences from the FP64 context: the J floor is 1e-30·J0 instead of Jeps;
+ * Δρu starts each step at zero (the step resets it, as m1lya_step does); the
+ * frequency solve is a cyclic reduction instead of the pivoted gtsv.
+ * Field ids are enum m1lya_field; a batch holds J, H_R, H_Z, ALPHA_F, HVOIGT,
+ * PHI_NU, ALPHA_LYA, ALPHA_C, PD, JS, T, DNU_D, the velocity gradients,
+ * UR_OVER_R, UR_STAR, UZ_STAR (upload) and J, H_R, H_Z, F_LYA_R, F_LYA_Z
+ * (download).  Velocity fields default to zero. */
+typedef struct m1lya_batch m1lya_batch;
+
+int  m1lya_batch_create(int n_clouds, const int *n_nu, const int *n_r, const int *n_z, m1lya_batch **out);
+void m1lya_batch_destroy(m1lya_batch *b);
+int  m1lya_batch_last_error(const m1lya_batch *b, char *buf, size_t len);
+int  m1lya_batch_count(const m1lya_batch *b);
+/* Per-cloud setup, as m1lya_set_geometry / m1lya_set_freq_grid. */
+int  m1lya_batch_set_geometry(m1lya_batch *b, int cloud, const m1lya_geometry *geometry);
+int  m1lya_batch_set_freq_grid(m1lya_batch *b, int cloud, const m1lya_freq_grid *grid);
+/* J0 (default 1): J and H are stored as J/J0, H/J0.  A resident state keeps its
+ * physical value when J0 changes.  Choose J0 near the largest J of the cloud. */
+int  m1lya_batch_set_scale(m1lya_batch *b, int cloud, double J0);
+int  m1lya_batch_get_scale(const m1lya_batch *b, int cloud, double *J0);
+int  m1lya_batch_upload(m1lya_batch *b, int cloud, int field, const double *host, size_t count);
+int  m1lya_batch_download(m1lya_batch *b, int cloud, int field, double *host, size_t count);
+/* c̃, Δt, Z_center and the boundary conditions of one cloud's steps. */
+int  m1lya_batch_set_step(m1lya_batch *b, int cloud, double c_tilde, double dt, double z_center,
+                          int bc_r_right, int bc_z_bot, int bc_z_top);
+/* n_steps steps of every cloud.  history (may be NULL) receives
+ * [n_steps][n_clouds][8] step scalars laid out as m1lya_step's out. */
+int  m1lya_batch_run_steps(m1lya_batch *b, int n_steps, double *history);
+/* [n_clouds][8] scalars of the last step. */
+int  m1lya_batch_scalars(m1lya_batch *b, double *out);
+/* [n_clouds][2] active-bin ranges the next step uses (0-based, inclusive). */
+int  m1lya_batch_active_bins(m1lya_batch *b, int *out);
+/* Tile shape: cells per wavefront segment along r, wavefronts per block (1..4).
+ * The state after a step does not depend on the tile shape. */
+int  m1lya_batch_set_tiles(m1lya_batch *b, int seg_len, int waves_per_block);
+int  m1lya_batch_get_tiles(const m1lya_batch *b, int *seg_len, int *waves_per_block);
+/* Times each candidate tile shape over trial_steps steps and keeps the fastest;
+ * the state is restored afterwards.  Outputs may be NULL. */
+int  m1lya_batch_autotune(m1lya_batch *b, int trial_steps, int *seg_len, int *waves_per_block, double *best_ms);
+/* on != 0: steps replay a captured hipGraph (one per state-buffer parity). */
+int  m1lya_batch_set_graph(m1lya_batch *b, int on);
+int  m1lya_batch_synchronize(m1lya_batch *b);
+int  m1lya_batch_set_profiling(m1lya_batch *b, int on);
+int  m1lya_batch_kernel_stats_count(m1lya_batch *b);
+int  m1lya_batch_kernel_stats(m1lya_batch *b, int index, char *name, size_t name_len,
+                              long *launches, double *total_ms);
+int  m1lya_batch_reset_kernel_stats(m1lya_batch *b);
+
 /* ---- instrumentation ------------------------------------------------- */
 /* When on, every kernel launch is bracketed by HIP events on the context's
  * stream; m1lya_kernel_stats() reports per-kernel launch counts and total

@@ -59,6 +59,12 @@ EXPORTED = [
     "m1lya_update_f_mf_l_gravity", "m1lya_compute_f_esc", "m1lya_reset_drhou", "m1lya_update_e_lya",
     "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_set_profiling",
     "m1lya_kernel_stats_count", "m1lya_kernel_stats", "m1lya_reset_kernel_stats", "m1lya_selftest_math",
+    "m1lya_batch_create", "m1lya_batch_destroy", "m1lya_batch_last_error", "m1lya_batch_count",
+    "m1lya_batch_set_geometry", "m1lya_batch_set_freq_grid", "m1lya_batch_set_scale", "m1lya_batch_get_scale",
+    "m1lya_batch_upload", "m1lya_batch_download", "m1lya_batch_set_step", "m1lya_batch_run_steps",
+    "m1lya_batch_scalars", "m1lya_batch_active_bins", "m1lya_batch_set_tiles", "m1lya_batch_get_tiles",
+    "m1lya_batch_autotune", "m1lya_batch_set_graph", "m1lya_batch_synchronize", "m1lya_batch_set_profiling",
+    "m1lya_batch_kernel_stats_count", "m1lya_batch_kernel_stats", "m1lya_batch_reset_kernel_stats",
 ]
 
 _lib = None
@@ -76,6 +82,7 @@ def load() -> C.CDLL:
     lib = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     i, d, sz = C.c_int, C.c_double, C.c_size_t
+    ip = C.POINTER(C.c_int)
     sig = {
         "m1lya_abi_version": (i, []),
         "m1lya_create": (i, [i, i, i, i, C.POINTER(vp)]),
@@ -106,6 +113,29 @@ def load() -> C.CDLL:
         "m1lya_kernel_stats": (i, [vp, i, C.c_char_p, sz, C.POINTER(C.c_long), _dp]),
         "m1lya_reset_kernel_stats": (i, [vp]),
         "m1lya_selftest_math": (i, [i, _dp, _dp, _dp, sz]),
+        "m1lya_batch_create": (i, [i, ip, ip, ip, C.POINTER(vp)]),
+        "m1lya_batch_destroy": (None, [vp]),
+        "m1lya_batch_last_error": (i, [vp, C.c_char_p, sz]),
+        "m1lya_batch_count": (i, [vp]),
+        "m1lya_batch_set_geometry": (i, [vp, i, C.POINTER(Geometry)]),
+        "m1lya_batch_set_freq_grid": (i, [vp, i, C.POINTER(FreqGrid)]),
+        "m1lya_batch_set_scale": (i, [vp, i, d]),
+        "m1lya_batch_get_scale": (i, [vp, i, _dp]),
+        "m1lya_batch_upload": (i, [vp, i, i, _dp, sz]),
+        "m1lya_batch_download": (i, [vp, i, i, _dp, sz]),
+        "m1lya_batch_set_step": (i, [vp, i, d, d, d, i, i, i]),
+        "m1lya_batch_run_steps": (i, [vp, i, _dp]),
+        "m1lya_batch_scalars": (i, [vp, _dp]),
+        "m1lya_batch_active_bins": (i, [vp, ip]),
+        "m1lya_batch_set_tiles": (i, [vp, i, i]),
+        "m1lya_batch_get_tiles": (i, [vp, ip, ip]),
+        "m1lya_batch_autotune": (i, [vp, i, ip, ip, _dp]),
+        "m1lya_batch_set_graph": (i, [vp, i]),
+        "m1lya_batch_synchronize": (i, [vp]),
+        "m1lya_batch_set_profiling": (i, [vp, i]),
+        "m1lya_batch_kernel_stats_count": (i, [vp]),
+        "m1lya_batch_kernel_stats": (i, [vp, i, C.c_char_p, sz, C.POINTER(C.c_long), _dp]),
+        "m1lya_batch_reset_kernel_stats": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -134,4 +164,11 @@ def check(rc: int, handle=None) -> None:
     if rc != OK:
         buf = C.create_string_buffer(1024)
         load().m1lya_last_error(handle, buf, 1024)
+        raise M1LyaError(rc, buf.value.decode(errors="replace"))
+
+
+def check_batch(rc: int, handle=None) -> None:
+    if rc != OK:
+        buf = C.create_string_buffer(1024)
+        load().m1lya_batch_last_error(handle, buf, 1024)
         raise M1LyaError(rc, buf.value.decode(errors="replace"))

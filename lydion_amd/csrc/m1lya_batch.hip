@@ -1,0 +1,1587 @@
+// m1lya_batch.hip -- the mixed-precision, batched M1 Lya step (m1lya_batch_* in
+// include/m1lya.h): kernels and C ABI.
+//
+// A batch holds clouds of different sizes (N_r, N_Z >= 3, 3 <= N_ν <= 128).  Their
+// state J, H_r, H_Z is stored in FP32, normalised per cloud by a scale J0.  One step
+// of every cloud is one launch of k_step plus two small reduction launches:
+//
+//   * one wavefront marches along r over a segment of one cell row; lanes hold the
+//     frequency bins (lane l owns bins lR .. lR+R-1, R = 1 for N_ν <= 64, else 2);
+//   * FP32 flux sweep + M1 closure (src/AdvectJandH.jl:526-1390, src/M1Closure.jl):
+//     the r interface between consecutive cells of the segment is evaluated once
+//     and carried in registers, the two Z interfaces of each cell in place;
+//   * FP64 emission / absorption (src/ConstantsAndFunctions.jl:483-573), frequency
+//     factors (src/FreqFactors.jl:142-278; each bin's exponentials are evaluated
+//     once and passed to the neighbouring rows by lane shuffles) and the frequency
+//     solve of src/UpdateLyaJandH.jl:178-391 as a wavefront-parallel cyclic
+//     reduction: one local level over the lane's bin pair, then parallel cyclic
+//     reduction across the 64 lanes;
+//   * H rescale and causality, FP32 store into the other state buffer;
+//   * Neumaier-compensated FP64 accumulation of M_F, L, f_esc and the active-bin
+//     energies: per block (k_step), per cloud (k_reduce), then k_final.
+//
+// Work items (one per wavefront) and partial-sum rows (one per block) cover every
+// cloud of the batch, so uneven clouds share one launch.  The three launches of a
+// step can be captured into a hipGraph per buffer parity; the segment length and
+// the number of wavefronts per block are chosen by a timing autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/m1lya.h"
+#include "m1lya_fast.h"
+
+#pragma clang fp contract(fast)
+
+namespace mb {
+
+constexpr double kC = 2.99792458e10;
+constexpr double kInvC = 1.0 / kC;
+constexpr double kFourPi = 4.0 * 3.141592653589793;
+constexpr double kHoverKB = 6.626069573e-27 / 1.380648813e-16;
+constexpr double kJeps = 1e-30;          // J floor in units of J0 (m1f::kJeps32)
+constexpr double kHeps2 = 1e-200;        // |H|^2 floor of the cell-centred closure
+constexpr double kEpsActive = 1e-10;     // FindActiveBins threshold
+constexpr double kPmin = 1e-3, kPmax = 40.0;
+constexpr int kMaxBins = 128;
+constexpr int kNScal = 5;                        // MF, L, net, out, esc
+constexpr int kPartStride = 2 * kMaxBins + 8;    // e1[N_ν], e10[N_ν], scalars
+constexpr int kMaxWaves = 4;
+
+// Per-cell inputs of the step, one 160-byte record per cell (read with scalar loads).
+struct CellRec {
+    double alpha, alpha_c, pd, js, T, dnuD, dV, rcom, zcom;
+    double durdr, durdz, duzdr, duzdz, urr;
+    double A_rout, A_ztop, A_zbot;   // areas of the cell's faces on the outer r / top / bottom boundary, else 0
+    float gRl, gRr, gZl, gZr;        // A/ΔV of the four faces
+    float inv_r, pad;
+};
+static_assert(sizeof(CellRec) == 160, "CellRec layout");
+
+// Per-bin constants of the frequency factors.
+struct BinRec {
+    double nuC, dnu, inv_dnu, inv_nuC;
+    double half_R, half_L;   // ν_{k+1} - νC_k and νC_k - ν_k
+};
+
+struct Cloud {
+    int nn, nr, nz, R;
+    long long o3, oc, ofr, ofz, obin, oivr, oivz;
+    int blk0, blk1;                // this cloud's partial rows (= its blocks)
+    int kcore_min, kcore_max, recoil, hll;
+    int bc_r_right, bc_z_bot, bc_z_top, has_vel;
+    float cdt, cdtc;               // c̃Δt and c̃Δt/c for the FP32 flux sweep
+    double c_tilde, dt, z_center, J0, alpha_floor;
+};
+
+struct Seg {
+    int cloud, j, i0, i1;          // cells [i0, i1) of row j; cloud < 0: idle wavefront
+};
+
+struct Ctrl {
+    double* hist;                  // [cap][n_clouds][8]
+    int cap, step;
+};
+
+struct Arrays {
+    float* J[2];
+    float* Hr[2];
+    float* Hz[2];
+    float *aF, *Hv, *phi;
+    const CellRec* rec;
+    const BinRec* bin;
+    float *uc_r, *uc_z;
+    float *ivr, *ivz;
+    double *f_r, *f_z;
+};
+
+// ------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------
+struct KSum {   // Neumaier compensated sum
+    double s, c;
+    __device__ void add(double x)
+    {
+        const double t = s + x;
+        c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+__device__ __forceinline__ double wave_sum64(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// 1/x: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double drcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// range of logical blocks so that neighbouring rows share its L2.
+__device__ __forceinline__ int xcd_block(int b, int grid)
+{
+    const int per = grid >> 3;   // grid is a multiple of 8
+    return (b & 7) * per + (b >> 3);
+}
+
+// Ghost-aware FP32 loads (the ghost values the reference builds at each boundary):
+//   r axis (i < 0): mirror, H_r negated                  src/AdvectJandH.jl:666-673
+//   r outer (i >= N_r): :Reflect mirror / :Outflow copy with H_r = max(H_r,0)  :853-871
+//   Z bottom (j < 0): :Reflect mirror / :Outflow copy with H_Z = min(H_Z,0)    :1000-1018
+//   Z top (j >= N_Z): :Reflect mirror / :Outflow copy with H_Z = max(H_Z,0)    :1185-1203
+__device__ __forceinline__ void ld_r(const float* J, const float* Hr, const float* Hz, int nn, int nr, int j,
+                                     int k, int c, int bc_right, float& vj, float& vr, float& vz)
+{
+    int m;
+    float sr = 1.0f;
+    bool outflow = false;
+    if (c < 0) {
+        m = -c - 1; sr = -1.0f;
+    } else if (c < nr) {
+        m = c;
+    } else if (bc_right == M1LYA_REFLECT) {
+        m = 2 * nr - 1 - c; sr = -1.0f;
+    } else {
+        m = nr - 1; outflow = true;
+    }
+    const size_t idx = ((size_t)j * nr + m) * nn + k;
+    vj = J[idx];
+    const float h = Hr[idx];
+    vr = outflow ? m1f::fmaxf_(h, 0.0f) : sr * h;
+    vz = Hz[idx];
+}
+
+__device__ __forceinline__ void ld_z(const float* J, const float* Hr, const float* Hz, int nn, int nr, int nz,
+                                     int i, int k, int c, int bc_bot, int bc_top, float& vj, float& vr,
+                                     float& vz)
+{
+    int m, mode = 0;   // 0 plain, 1 negate, 2 min(H_Z,0), 3 max(H_Z,0)
+    if (c < 0) {
+        if (bc_bot == M1LYA_REFLECT) { m = -c - 1; mode = 1; }
+        else { m = 0; mode = 2; }
+    } else if (c < nz) {
+        m = c;
+    } else if (bc_top == M1LYA_REFLECT) {
+        m = 2 * nz - 1 - c; mode = 1;
+    } else {
+        m = nz - 1; mode = 3;
+    }
+    const size_t idx = ((size_t)m * nr + i) * nn + k;
+    vj = J[idx];
+    vr = Hr[idx];
+    const float h = Hz[idx];
+    vz = (mode == 0) ? h : (mode == 1) ? -h : (mode == 2) ? m1f::fminf_(h, 0.0f) : m1f::fmaxf_(h, 0.0f);
+}
+
+// Compute_D_Edd (src/M1Closure.jl:46-83) in FP64 with the cancellation-free a, b.
+__device__ inline void D_edd64(double Hr, double Hz, double J, double& Drr, double& Dzz, double& Drz)
+{
+    const double Hm2 = Hr * Hr + Hz * Hz;
+    if (J <= kJeps || Hm2 <= kHeps2 || Hm2 <= 1e-20 * J * J) {
+        Drr = 1.0 / 3.0; Dzz = 1.0 / 3.0; Drz = 0.0;
+        return;
+    }
+    const double Hm = sqrt(Hm2);
+    const double iH = drcp(Hm);
+    double f = Hm * drcp(J);
+    f = f < 1e-10 ? 1e-10 : (f > 1.0 - 1e-10 ? 1.0 - 1e-10 : f);
+    double a, b;
+    m1f::levermore_ab64(f, a, b);
+    const double nr = Hr * iH, nz = Hz * iH;
+    Drr = a + b * nr * nr;
+    Dzz = a + b * nz * nz;
+    Drz = b * nr * nz;
+}
+
+// 1 - ϕ and χ of the exponential fitting, both sides of one P (src/FreqFactors.jl:65-118):
+// oL = exp(-P), xL = (1 - oL)/P, oR = exp(P), xR = (1 - oR)/P; P is already clamped to ±40.
+__device__ inline void fit_both(double P, double& oL, double& xL, double& oR, double& xR)
+{
+    if (fabs(P) < kPmin) {
+        const double h = 0.5 * (P * P);
+        oL = 1.0 - P + h; xL = 1.0 - 0.5 * P;
+        oR = 1.0 + P + h; xR = -1.0 - 0.5 * P;
+        return;
+    }
+    const double e = exp(-P);
+    const double iP = drcp(P);
+    oL = e;
+    xL = (1.0 - e) * iP;
+    oR = drcp(e);
+    xR = (1.0 - oR) * iP;
+}
+
+__device__ inline void fit_right(double P, double& oR, double& xR)
+{
+    if (fabs(P) < kPmin) {
+        oR = 1.0 + P + 0.5 * (P * P);
+        xR = -1.0 - 0.5 * P;
+        return;
+    }
+    oR = exp(P);
+    xR = (1.0 - oR) * drcp(P);
+}
+
+// Bins of one lane: R consecutive bins starting at lane*R.
+struct LaneBins {
+    int kb[2];
+    bool val[2], act[2];
+};
+
+// ------------------------------------------------------------------------
+// k_step: one full M1 step of every cloud of the batch.
+// Block = wpb wavefronts, one row segment each; all segments of a block belong
+// to one cloud, whose partial row the block writes.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+                                                 const Seg* __restrict__ segs, int nblk, int wpb, int parity,
+                                                 const int* __restrict__ krange, double* __restrict__ part)
+{
+    __shared__ double sh[kMaxWaves][kPartStride];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= nblk) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Seg sg = segs[(size_t)lb * wpb + wave];
+    const int cidx = segs[(size_t)lb * wpb].cloud;
+
+    double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
+    KSum mf{0, 0}, Ls{0, 0}, net{0, 0}, outw{0, 0}, esc{0, 0};
+
+    if (sg.cloud >= 0 && sg.i0 < sg.i1) {
+        const Cloud& cl = clouds[sg.cloud];
+        const int nn = cl.nn, nr = cl.nr, nz = cl.nz, R = cl.R, j = sg.j;
+        const int hll = cl.hll, bcr = cl.bc_r_right, bcb = cl.bc_z_bot, bct = cl.bc_z_top;
+        const float* J = A.J[parity] + cl.o3;
+        const float* Hr = A.Hr[parity] + cl.o3;
+        const float* Hz = A.Hz[parity] + cl.o3;
+        float* Jw = A.J[parity ^ 1] + cl.o3;
+        float* Hrw = A.Hr[parity ^ 1] + cl.o3;
+        float* Hzw = A.Hz[parity ^ 1] + cl.o3;
+        const float* aF = A.aF + cl.o3;
+        const float* Hvp = A.Hv + cl.o3;
+        const float* php = A.phi + cl.o3;
+        const float* ivr = A.ivr + cl.oivr;
+        const float* ivz = A.ivz + cl.oivz;
+        const float* ucr = A.uc_r + cl.ofr;
+        const float* ucz = A.uc_z + cl.ofz;
+        const CellRec* rec = A.rec + cl.oc;
+        const BinRec* bins = A.bin + cl.obin;
+        const int kmin = krange[2 * sg.cloud], kmax = krange[2 * sg.cloud + 1];
+
+        LaneBins L;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            L.kb[q] = lane * R + q;
+            L.val[q] = (q < R) && (L.kb[q] < nn);
+            L.act[q] = L.val[q] && L.kb[q] >= kmin && L.kb[q] <= kmax;
+        }
+
+        const double ct = cl.c_tilde, dt = cl.dt, cdt = ct * dt, dw = 1.0 / cdt;
+        const double inv_J0 = 1.0 / cl.J0;
+        const double fpc_ict = (kFourPi / kC) / ct * cl.J0;
+        const double inv_dt = 1.0 / dt;
+        const float cdtf = cl.cdt, cdtcf = cl.cdtc;
+        const int i0 = sg.i0;
+
+        // r window: cells (i-1, i, i+1, i+2) of the current step of the march;
+        // FL / kimR: flux and K_ϕϕ of the left interface of cell i.
+        float wJ[2][4], wR[2][4], wZ[2][4], FL[2][3], kimR[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                wJ[q][m] = 0.f; wR[q][m] = 0.f; wZ[q][m] = 0.f;
+                // cells i0-2, i0-1, i0 in slots 1..3: the first shift and load of the
+                // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1)
+                if (L.val[q] && m >= 1)
+                    ld_r(J, Hr, Hz, nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m], wR[q][m], wZ[q][m]);
+            }
+            FL[q][0] = FL[q][1] = FL[q][2] = 0.f;
+            kimR[q] = 0.f;
+        }
+
+        // i = i0-1 only evaluates the left interface of cell i0.
+        for (int i = i0 - 1; i < sg.i1; ++i) {
+            const bool first = (i < i0);
+            const size_t cell = (size_t)j * nr + i;   // unused when first
+            const size_t base = cell * nn;
+            CellRec cr;
+            if (!first) cr = rec[cell];
+
+            double jo[2], hr[2], hz[2];
+            // ================= flux sweep (FP32) =================
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
+                if (!L.val[q]) continue;
+                const int k = L.kb[q];
+                wJ[q][0] = wJ[q][1]; wJ[q][1] = wJ[q][2]; wJ[q][2] = wJ[q][3];
+                wR[q][0] = wR[q][1]; wR[q][1] = wR[q][2]; wR[q][2] = wR[q][3];
+                wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
+                ld_r(J, Hr, Hz, nn, nr, j, k, i + 2, bcr, wJ[q][3], wR[q][3], wZ[q][3]);
+                const float Js = wJ[q][1], Hrs = wR[q][1], Hzs = wZ[q][1];
+                float Jn = Js, Hrn = Hrs, Hzn = Hzs;
+                if (L.act[q]) {
+                    float dJ = 0.f, dHr = 0.f, dHz = 0.f;   // Z part of the flux divergence
+                    float FR0 = 0.f, FR1 = 0.f, FR2 = 0.f, kip = 0.f, kimN = 0.f;
+                    const int nside = first ? 1 : 3;
+#pragma unroll 1
+                    for (int side = 0; side < nside; ++side) {
+                        m1f::Stencil32 st;
+                        const float* iv;
+                        float aL, aR, u;
+                        int dir;
+                        if (side == 0) {
+                            // right r interface of cell i: cells (i, i+1, i+2, i-1)
+                            dir = 0;
+                            st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
+                            st.Hr[0] = wR[q][1]; st.Hr[1] = wR[q][2]; st.Hr[2] = wR[q][3]; st.Hr[3] = wR[q][0];
+                            st.Hz[0] = wZ[q][1]; st.Hz[1] = wZ[q][2]; st.Hz[2] = wZ[q][3]; st.Hz[3] = wZ[q][0];
+                            iv = ivr + (size_t)(i + 1) * 5;
+                            aL = aF[((size_t)j * nr + max(i, 0)) * nn + k];
+                            aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
+                            u = ucr[(size_t)j * (nr + 1) + i + 1];
+                        } else {
+                            // Z interface g (bottom g = j, top g = j+1): cells (g-1, g, g+1, g-2)
+                            dir = 1;
+                            const int g = j + side - 1;
+                            const int cells[4] = {g - 1, g, g + 1, g - 2};
+#pragma unroll
+                            for (int m = 0; m < 4; ++m)
+                                ld_z(J, Hr, Hz, nn, nr, nz, i, k, cells[m], bcb, bct, st.J[m], st.Hr[m], st.Hz[m]);
+                            iv = ivz + (size_t)g * 5;
+                            aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
+                            aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+                            u = ucz[(size_t)g * nr + i];
+                        }
+                        const m1f::Face32 f = m1f::interface_flux(dir, hll, st, iv, aL, aR, u);
+                        if (side == 0) {
+                            FR0 = f.FJ; FR1 = f.FHr; FR2 = f.FHz;
+                            kip = f.kphi_L;
+                            kimN = (i + 1 == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
+                        } else {
+                            const float gz = (side == 1) ? -cr.gZl : cr.gZr;
+                            dJ += gz * f.FJ;
+                            dHr += gz * f.FHr;
+                            dHz += gz * f.FHz;
+                        }
+                    }
+                    if (!first) {
+                        // cell update: src/AdvectJandH.jl:1319-1379
+                        const float gRl = cr.gRl, gRr = cr.gRr;
+                        Jn = Js - cdtf * ((gRr * FR0 - gRl * FL[q][0]) + dJ);
+                        Jn = m1f::fmaxf_(Jn, m1f::kJeps32);
+                        Hzn = Hzs - cdtf * ((gRr * FR2 - gRl * FL[q][2]) + dHz);
+                        Hrn = Hrs - cdtf * ((gRr * FR1 - gRl * FL[q][1]) + dHr);
+                        Hrn += cdtf * (0.5f * (kimR[q] + kip)) * cr.inv_r;
+                        if (cl.has_vel) {
+                            const float durdr = (float)cr.durdr, durdz = (float)cr.durdz;
+                            const float duzdr = (float)cr.duzdr, duzdz = (float)cr.duzdz, urr = (float)cr.urr;
+                            const m1f::Edd32 D = m1f::D_edd(Hrs, Hzs, Js);
+                            const float Dpp = m1f::fmaxf_(1.0f - D.rr - D.zz, 0.0f);
+                            const float Ddu = D.rr * durdr + D.rz * (durdz + duzdr) + D.zz * duzdz + Dpp * urr;
+                            Hrn += -cdtcf * (Hrs * durdr + Hzs * durdz);
+                            Hzn += -cdtcf * (Hrs * duzdr + Hzs * duzdz);
+                            Jn = m1f::fmaxf_(Jn - cdtcf * Ddu * Js, m1f::kJeps32);
+                        }
+                        // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
+                        const float Hm2 = Hrn * Hrn + Hzn * Hzn;
+                        if (Hm2 > Jn * Jn) {
+                            const float fac = Jn * m1f::rcp(m1f::fsqrt(Hm2));
+                            Hrn *= fac;
+                            Hzn *= fac;
+                        }
+                    }
+                    FL[q][0] = FR0; FL[q][1] = FR1; FL[q][2] = FR2;
+                    kimR[q] = kimN;
+                }
+                jo[q] = Jn; hr[q] = Hrn; hz[q] = Hzn;
+            }
+            if (first) continue;
+
+            // ================= emission / absorption (FP64) =================
+            // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462), J in units of J0
+            const double al_pd = cr.alpha * cr.pd, src = cr.js * ct * dt * inv_J0;
+            double hv[2] = {1.0, 1.0};
+            double sr = 0.0, sz = 0.0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (!L.val[q]) continue;
+                const size_t idx = base + L.kb[q];
+                const double h = Hvp[idx];
+                hv[q] = h;
+                const double aJ = cr.alpha_c + al_pd * h;
+                const double jn = (jo[q] + src * (double)php[idx]) * drcp(1.0 + aJ * cdt);
+                const double iH = drcp(1.0 + (double)aF[idx] * cdt);
+                const double hrn = hr[q] * iH, hzn = hz[q] * iH;
+                const double dk = bins[L.kb[q]].dnu;
+                sr += dk * (hr[q] - hrn);
+                sz += dk * (hz[q] - hzn);
+                jo[q] = jn; hr[q] = hrn; hz[q] = hzn;
+            }
+            sr = wave_sum64(sr);
+            sz = wave_sum64(sz);
+            {
+                // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
+                const double fr = fpc_ict * sr * inv_dt, fz = fpc_ict * sz * inv_dt;
+                if (lane == 0) {
+                    A.f_r[cl.oc + cell] = fr;
+                    A.f_z[cl.oc + cell] = fz;
+                }
+                mf.add(cr.dV * sqrt(fr * fr + fz * fz));
+                Ls.add(kFourPi * cr.dV * cr.js);
+                const double ddr = cr.rcom, ddz = cr.zcom - cl.z_center;
+                const double dist = sqrt(ddr * ddr + ddz * ddz);
+                double frad = 0.0;
+                if (dist > 0.0) frad = (fr * ddr + fz * ddz) / dist;
+                net.add(cr.dV * frad);
+                outw.add(cr.dV * (frad > 0.0 ? frad : 0.0));
+            }
+
+            // ================= frequency factors (FP64) =================
+            // Per bin m: K_m, C_m and the fitting values of P_m (right half-width, the
+            // "dk" of row m and "dkm" of row m+1) and P'_m (left half-width, the "dkp"
+            // of row m-1).  Row k takes bin k-1's (A, oL) and bin k+1's (B', oR').
+            const bool vel = cl.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
+                                            cr.duzdz != 0.0 || cr.urr != 0.0);
+            const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
+            const double R0 = cl.recoil ? kHoverKB / cr.T : 0.0;
+            double Aq[2], Bq[2], oLq[2], oRq[2], Bpq[2], oRpq[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                Aq[q] = 0.0; Bq[q] = 0.0; oLq[q] = 1.0; oRq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
+                if (!L.val[q]) continue;
+                const BinRec bk = bins[L.kb[q]];
+                double VEL = 0.0;
+                if (vel) {
+                    double Dr, Dz, Dx;
+                    D_edd64(hr[q], hz[q], jo[q], Dr, Dz, Dx);
+                    double Dpp = 1.0 - Dr - Dz;
+                    Dpp = Dpp > 0.0 ? Dpp : 0.0;
+                    VEL = (Dr * cr.durdr + Dx * (cr.durdz + cr.duzdr) + Dz * cr.duzdz + Dpp * cr.urr) * kInvC;
+                }
+                const double K = PREF * hv[q];
+                const double iK = drcp(K);
+                double C = VEL * bk.nuC;
+                if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
+                const double qR = bk.half_R * iK, qL = bk.half_L * iK;
+                const double P = fmin(fmax(C * qR, -kPmax), kPmax);
+                const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
+                double oL, xL, oR, xR, oRp, xRp;
+                fit_both(P, oL, xL, oR, xR);
+                if (Pp == P) {
+                    oRp = oR; xRp = xR;
+                } else {
+                    fit_right(Pp, oRp, xRp);
+                }
+                Aq[q] = xL * qR;
+                Bq[q] = xR * qR;
+                oLq[q] = oL;
+                oRq[q] = oR;
+                Bpq[q] = xRp * qL;
+                oRpq[q] = oRp;
+            }
+            const int R1 = R - 1;
+            const double A_prev = __shfl_up(R1 ? Aq[1] : Aq[0], 1, 64);
+            const double oL_prev = __shfl_up(R1 ? oLq[1] : oLq[0], 1, 64);
+            const double Bp_next = __shfl_down(Bpq[0], 1, 64), oRp_next = __shfl_down(oRpq[0], 1, 64);
+            double ra[2], rb[2], rc[2], rd[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
+                const int k = L.kb[q];
+                if (!L.val[q] || k < 1 || k > nn - 2) continue;
+                const double Am = (q == 0) ? A_prev : Aq[0];
+                const double oLm = (q == 0) ? oL_prev : oLq[0];
+                const bool last = (q == R1);
+                const double Bpp = last ? Bp_next : Bpq[1];
+                const double oRpp = last ? oRp_next : oRpq[1];
+                const double idn = bins[k].inv_dnu;
+                const double den_kp = idn * drcp(Aq[q] - Bpp);
+                const double den_km = idn * drcp(Am - Bq[q]);
+                const double Fkp = oRpp * den_kp;
+                const double Fkm = oLm * den_km;
+                const double Fk = oLq[q] * den_kp + oRq[q] * den_km;
+                // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215); gtsv drops
+                // the couplings to bins 0 and N_ν-1, which keep their value
+                ra[q] = (k == 1) ? 0.0 : Fkm;
+                rc[q] = (k == nn - 2) ? 0.0 : Fkp;
+                rb[q] = -(Fk + dw);
+                rd[q] = -dw * jo[q];
+            }
+
+            // ================= tridiagonal solve: CR level + PCR over the lanes =================
+            double a, b, c, d, ua = 0.0, uc = 0.0, ud = 0.0;
+            if (R == 2) {
+                // eliminate the odd bin of each lane's pair into the even rows
+                const double ib1 = drcp(rb[1]);
+                ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
+                double pua = __shfl_up(ua, 1, 64), puc = __shfl_up(uc, 1, 64), pud = __shfl_up(ud, 1, 64);
+                if (lane == 0) { pua = 0.0; puc = 0.0; pud = 0.0; }
+                b = rb[0] - ra[0] * puc - rc[0] * ua;
+                d = rd[0] - ra[0] * pud - rc[0] * ud;
+                a = -ra[0] * pua;
+                c = -rc[0] * uc;
+            } else {
+                a = ra[0]; b = rb[0]; c = rc[0]; d = rd[0];
+            }
+#pragma unroll
+            for (int s = 1; s < 64; s <<= 1) {
+                const double ib = drcp(b);
+                const double u = a * ib, v = c * ib, w = d * ib;
+                double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
+                double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
+                if (lane < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
+                if (lane + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
+                const double nb = b - a * vb - c * uf;
+                const double nd = d - a * wb - c * wf;
+                a = -a * ub;
+                c = -c * vf;
+                b = nb;
+                d = nd;
+            }
+            double xs[2];
+            xs[0] = d * drcp(b);
+            xs[1] = 0.0;
+            if (R == 2) {
+                double xn = __shfl_down(xs[0], 1, 64);
+                if (lane == 63) xn = 0.0;
+                xs[1] = ud - ua * xs[0] - uc * xn;
+            }
+
+            // ================= H correction, store, reductions =================
+            const bool edge = (cr.A_rout != 0.0) || (cr.A_ztop != 0.0) || (cr.A_zbot != 0.0 && bcb == M1LYA_OUTFLOW);
+            const double Azb = (bcb == M1LYA_OUTFLOW) ? cr.A_zbot : 0.0;
+            double escl = 0.0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (!L.val[q]) continue;
+                const int k = L.kb[q];
+                // J floor, H rescale and causality: src/UpdateLyaJandH.jl:335-379
+                const double jn = (k >= 1 && k <= nn - 2) ? xs[q] : jo[q];
+                const double Jk = jn > kJeps ? jn : kJeps;
+                const double ratio = Jk * drcp(jo[q] > kJeps ? jo[q] : kJeps);
+                double h_r = hr[q] * ratio, h_z = hz[q] * ratio;
+                const double Hm2 = h_r * h_r + h_z * h_z;
+                if (Hm2 > Jk * Jk) {
+                    const double s = Jk * drcp(sqrt(Hm2));
+                    h_r *= s;
+                    h_z *= s;
+                }
+                const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
+                const size_t idx = base + k;
+                Jw[idx] = Jf;
+                Hrw[idx] = hrf;
+                Hzw[idx] = hzf;
+                const double Jd = Jf, dk = bins[k].dnu;
+                e1[q] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
+                e10[q] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
+                // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
+                if (edge)
+                    escl += dk * kFourPi *
+                            (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
+                             Azb * fmax(-(double)hzf, 0.0));
+            }
+            if (edge) esc.add(wave_sum64(escl) * cl.J0);
+        }
+    }
+
+    // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
+    const Cloud& cb = clouds[cidx];
+    const int nnb = cb.nn, Rb = cb.R;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k = lane * Rb + q;
+        if (q < Rb && k < nnb) {
+            sh[wave][k] = e1[q] * cb.J0;
+            sh[wave][nnb + k] = e10[q] * cb.J0;
+        }
+    }
+    if (lane == 0) {
+        sh[wave][2 * nnb + 0] = mf.value();
+        sh[wave][2 * nnb + 1] = Ls.value();
+        sh[wave][2 * nnb + 2] = net.value();
+        sh[wave][2 * nnb + 3] = outw.value();
+        sh[wave][2 * nnb + 4] = esc.value();
+    }
+    __syncthreads();
+    const int ncol = 2 * nnb + kNScal;
+    for (int t = threadIdx.x; t < ncol; t += blockDim.x) {
+        KSum s{0, 0};
+        for (int w = 0; w < wpb; ++w) s.add(sh[w][t]);
+        part[(size_t)lb * kPartStride + t] = s.value();
+    }
+}
+
+// Active-bin energies straight from the state (before the first step after an
+// upload): block rows in k_step's format, scalars zero.
+__global__ __launch_bounds__(256) void k_active_rows(const Cloud* __restrict__ clouds, Arrays A,
+                                                     const Seg* __restrict__ segs, int nblk, int wpb, int parity,
+                                                     double* __restrict__ part)
+{
+    __shared__ double sh[kMaxWaves][kPartStride];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= nblk) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Seg sg = segs[(size_t)lb * wpb + wave];
+    const Cloud& cb = clouds[segs[(size_t)lb * wpb].cloud];
+    const int nnb = cb.nn, R = cb.R;
+    double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
+    if (sg.cloud >= 0) {
+        const float* J = A.J[parity] + cb.o3;
+        for (int i = sg.i0; i < sg.i1; ++i) {
+            const size_t cell = (size_t)sg.j * cb.nr + i;
+            const double dV = A.rec[cb.oc + cell].dV;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int k = lane * R + q;
+                if (q >= R || k >= nnb) continue;
+                const double Jd = J[cell * nnb + k];
+                const double dk = A.bin[cb.obin + k].dnu;
+                e1[q] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * dV;
+                e10[q] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * dV;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k = lane * R + q;
+        if (q < R && k < nnb) {
+            sh[wave][k] = e1[q] * cb.J0;
+            sh[wave][nnb + k] = e10[q] * cb.J0;
+        }
+    }
+    if (lane < kNScal) sh[wave][2 * nnb + lane] = 0.0;
+    __syncthreads();
+    const int ncol = 2 * nnb + kNScal;
+    for (int t = threadIdx.x; t < ncol; t += blockDim.x) {
+        KSum s{0, 0};
+        for (int w = 0; w < wpb; ++w) s.add(sh[w][t]);
+        part[(size_t)lb * kPartStride + t] = s.value();
+    }
+}
+
+// Per-cloud sums of the block rows: lanes over columns, wavefronts over rows,
+// Neumaier sums combined in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds, const double* __restrict__ part,
+                                                double* __restrict__ red)
+{
+    __shared__ double ss[4][64], sc[4][64];
+    const Cloud& cl = clouds[blockIdx.y];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    const int ncol = 2 * cl.nn + kNScal;
+    if ((int)blockIdx.x * 64 >= ncol) return;
+    KSum acc{0, 0};
+    if (col < ncol)
+        for (int b = cl.blk0 + wave; b < cl.blk1; b += 4) acc.add(part[(size_t)b * kPartStride + col]);
+    ss[wave][lane] = acc.s;
+    sc[wave][lane] = acc.c;
+    __syncthreads();
+    if (wave == 0 && col < ncol) {
+        KSum t{ss[0][lane], sc[0][lane] + sc[1][lane] + sc[2][lane] + sc[3][lane]};
+        t.add(ss[1][lane]);
+        t.add(ss[2][lane]);
+        t.add(ss[3][lane]);
+        red[(size_t)blockIdx.y * kPartStride + col] = t.value();
+    }
+}
+
+// FindActiveBins (src/UpdateLyaJandH.jl:47-97) from the reduced energies.
+__device__ void active_range(const double* r, int nn, int kcmin, int kcmax, int* kr)
+{
+    double Etot = 0.0;
+    for (int k = 0; k < nn; ++k) Etot += r[k];
+    int kmn = nn - 1, kmx = 0;
+    for (int k = 0; k < nn; ++k)
+        if (r[nn + k] > kEpsActive * Etot) {
+            kmn = min(kmn, k);
+            kmx = max(kmx, k);
+        }
+    kr[0] = min(kmn, kcmin);
+    kr[1] = max(kmx, kcmax);
+}
+
+// Step scalars -- Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673)
+// and Compute_f_esc_Lyα_Cylindrical (:969-993) -- then the next step's active bins.
+// One block; thread 0 advances the history cursor after every cloud is written.
+__global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds, int n, const double* __restrict__ red,
+                                               int* __restrict__ krange, double* __restrict__ scalars,
+                                               Ctrl* __restrict__ ctrl, int active_only)
+{
+    const int step = ctrl->step;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const Cloud& cl = clouds[c];
+        const double* r = red + (size_t)c * kPartStride;
+        const int nn = cl.nn;
+        if (!active_only) {
+            double MF = r[2 * nn], L = r[2 * nn + 1], nt = r[2 * nn + 2], ow = r[2 * nn + 3];
+            const double es = r[2 * nn + 4];
+            if (L > 0.0) {
+                const double s = 1.0 / (L / kC);
+                MF *= s; nt *= s; ow *= s;
+            } else {
+                MF = 0.0; nt = 0.0; ow = 0.0;
+            }
+            const double Ltot = (cl.bc_z_bot == M1LYA_REFLECT) ? 2.0 * L : L;
+            double fesc = 0.0;
+            if (Ltot > 0.0) {
+                const double Lsim = (cl.bc_z_bot == M1LYA_REFLECT) ? 0.5 * Ltot : Ltot;
+                fesc = (Lsim > 0.0) ? es / Lsim : 0.0;
+            }
+            const double v[8] = {MF, nt, ow, Ltot, fesc, (double)krange[2 * c], (double)krange[2 * c + 1], 0.0};
+            for (int m = 0; m < 8; ++m) scalars[(size_t)c * 8 + m] = v[m];
+            if (ctrl->hist && step < ctrl->cap)
+                for (int m = 0; m < 8; ++m) ctrl->hist[((size_t)step * n + c) * 8 + m] = v[m];
+        }
+        active_range(r, nn, cl.kcore_min, cl.kcore_max, krange + 2 * c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && !active_only) ctrl->step = step + 1;
+}
+
+__global__ void k_scale(float* a, float* b, float* c, size_t n, float s)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) { a[t] *= s; b[t] *= s; c[t] *= s; }
+}
+
+}  // namespace mb
+
+// =========================================================================
+// Host side
+// =========================================================================
+using namespace mb;
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+struct BFail {
+    int code;
+};
+
+struct KStatB {
+    long launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace
+
+struct m1lya_batch {
+    int n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Cloud> cl;                 // host copy of the descriptors
+    std::vector<char> geom_set, grid_set;
+    Cloud* d_cl = nullptr;
+    bool cl_dirty = true;
+
+    size_t n3 = 0, nc = 0, nfr = 0, nfz = 0, nbin = 0, nivr = 0, nivz = 0;
+    std::vector<CellRec> h_rec;            // host mirror of the cell records
+    std::vector<BinRec> h_bin;
+    CellRec* d_rec = nullptr;
+    BinRec* d_bin = nullptr;
+    Arrays A{};
+    int parity = 0;
+    bool krange_valid = false;
+
+    // tile shape and the work / partial-row tables it implies
+    int seg_len = 16, wpb = 4;
+    std::vector<Seg> segs;
+    Seg* d_segs = nullptr;
+    size_t segs_cap = 0;
+    int nblk = 0, grid = 0;
+    double* part = nullptr;
+    size_t part_rows_cap = 0;
+    double* red = nullptr;
+    int* krange = nullptr;
+    double* scalars = nullptr;
+    Ctrl* ctrl = nullptr;
+    Ctrl h_ctrl{nullptr, 0, 0};
+    double* hist = nullptr;
+    int hist_cap = 0;
+
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+    std::vector<void*> allocs;
+    std::string err;
+    bool profiling = false;
+    std::map<std::string, KStatB> stats;
+};
+
+namespace {
+
+void bfail(m1lya_batch* b, int code, const std::string& msg)
+{
+    if (b) b->err = msg;
+    g_batch_error = msg;
+    throw BFail{code};
+}
+
+void bchk(m1lya_batch* b, hipError_t e, const char* what)
+{
+    if (e != hipSuccess) bfail(b, M1LYA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* balloc(m1lya_batch* b, size_t n)
+{
+    void* p = nullptr;
+    bchk(b, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+    b->allocs.push_back(p);
+    bchk(b, hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)), "hipMemset");
+    return (T*)p;
+}
+
+void bfree(m1lya_batch* b, void* p)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    b->allocs.erase(std::remove(b->allocs.begin(), b->allocs.end(), p), b->allocs.end());
+}
+
+template <class F>
+int bguard(m1lya_batch* b, F&& f)
+{
+    try {
+        f();
+        return M1LYA_OK;
+    } catch (const BFail& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        if (b) b->err = e.what();
+        g_batch_error = e.what();
+        return M1LYA_ERR_HIP;
+    }
+}
+
+void drop_graphs(m1lya_batch* b)
+{
+    for (int p = 0; p < 2; ++p)
+        if (b->gexec[p]) {
+            (void)hipGraphExecDestroy(b->gexec[p]);
+            b->gexec[p] = nullptr;
+        }
+}
+
+void check_cloud(m1lya_batch* b, int c)
+{
+    if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+    if (c < 0 || c >= b->n) bfail(b, M1LYA_ERR_ARG, "cloud index out of range");
+}
+
+// Work table of the current tile shape: per cloud, row segments of seg_len cells,
+// wpb of them (same i range, consecutive rows) per block; blocks never mix clouds.
+void build_tiles(m1lya_batch* b)
+{
+    std::vector<Seg> segs;
+    int blk = 0;
+    for (int c = 0; c < b->n; ++c) {
+        Cloud& cl = b->cl[c];
+        cl.blk0 = blk;
+        const int nseg_r = (cl.nr + b->seg_len - 1) / b->seg_len;
+        for (int j0 = 0; j0 < cl.nz; j0 += b->wpb)
+            for (int s = 0; s < nseg_r; ++s) {
+                for (int w = 0; w < b->wpb; ++w) {
+                    const int j = j0 + w;
+                    if (j < cl.nz)
+                        segs.push_back(Seg{c, j, s * b->seg_len, std::min(cl.nr, (s + 1) * b->seg_len)});
+                    else
+                        segs.push_back(Seg{-1, 0, 0, 0});
+                }
+                ++blk;
+            }
+        cl.blk1 = blk;
+    }
+    b->nblk = blk;
+    b->grid = (blk + 7) / 8 * 8;
+    if (segs.size() > b->segs_cap) {
+        bfree(b, b->d_segs);
+        b->d_segs = balloc<Seg>(b, segs.size());
+        b->segs_cap = segs.size();
+    }
+    if ((size_t)blk > b->part_rows_cap) {
+        bfree(b, b->part);
+        b->part = balloc<double>(b, (size_t)blk * kPartStride);
+        b->part_rows_cap = blk;
+    }
+    bchk(b, hipMemcpy(b->d_segs, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice), "segments");
+    b->segs = std::move(segs);
+    b->cl_dirty = true;
+    drop_graphs(b);
+}
+
+void sync_clouds(m1lya_batch* b)
+{
+    if (!b->cl_dirty) return;
+    bchk(b, hipMemcpy(b->d_cl, b->cl.data(), b->n * sizeof(Cloud), hipMemcpyHostToDevice), "cloud descriptors");
+    b->cl_dirty = false;
+}
+
+void push_records(m1lya_batch* b, int cloud)
+{
+    const Cloud& cl = b->cl[cloud];
+    const size_t ncell = (size_t)cl.nr * cl.nz;
+    bchk(b, hipMemcpy(b->d_rec + cl.oc, b->h_rec.data() + cl.oc, ncell * sizeof(CellRec), hipMemcpyHostToDevice),
+         "cell records");
+}
+
+void set_ctrl(m1lya_batch* b, double* hist, int cap)
+{
+    b->h_ctrl = Ctrl{hist, cap, 0};
+    bchk(b, hipMemcpy(b->ctrl, &b->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice), "ctrl");
+}
+
+// Brackets one launch with events when profiling (and waits for it).
+struct BProf {
+    m1lya_batch* b;
+    const char* name;
+    hipEvent_t e0{}, e1{};
+    BProf(m1lya_batch* b_, const char* n) : b(b_), name(n)
+    {
+        if (b->profiling) {
+            bchk(b, hipEventCreate(&e0), "hipEventCreate");
+            bchk(b, hipEventCreate(&e1), "hipEventCreate");
+            bchk(b, hipEventRecord(e0, b->stream), "hipEventRecord");
+        }
+    }
+    void done()
+    {
+        bchk(b, hipGetLastError(), name);
+        if (b->profiling) {
+            bchk(b, hipEventRecord(e1, b->stream), "hipEventRecord");
+            bchk(b, hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            bchk(b, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            KStatB& s = b->stats[name];
+            s.launches += 1;
+            s.total_ms += ms;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+};
+
+void launch_step_kernels(m1lya_batch* b, int parity)
+{
+    hipStream_t s = b->stream;
+    {
+        BProf p(b, "k_step");
+        hipLaunchKernelGGL(k_step, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
+                           parity, b->krange, b->part);
+        p.done();
+    }
+    {
+        BProf p(b, "k_reduce");
+        hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n), dim3(256), 0, s, b->d_cl, b->part,
+                           b->red);
+        p.done();
+    }
+    {
+        BProf p(b, "k_final");
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 0);
+        p.done();
+    }
+}
+
+void compute_krange(m1lya_batch* b)
+{
+    sync_clouds(b);
+    hipStream_t s = b->stream;
+    hipLaunchKernelGGL(k_active_rows, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs, b->nblk,
+                       b->wpb, b->parity, b->part);
+    bchk(b, hipGetLastError(), "k_active_rows");
+    hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n), dim3(256), 0, s, b->d_cl, b->part,
+                       b->red);
+    bchk(b, hipGetLastError(), "k_reduce");
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 1);
+    bchk(b, hipGetLastError(), "k_final");
+    b->krange_valid = true;
+}
+
+void require_ready(m1lya_batch* b)
+{
+    for (int c = 0; c < b->n; ++c) {
+        const std::string who = "cloud " + std::to_string(c);
+        if (!b->geom_set[c]) bfail(b, M1LYA_ERR_STATE, who + ": geometry not set");
+        if (!b->grid_set[c]) bfail(b, M1LYA_ERR_STATE, who + ": frequency grid not set");
+        if (!(b->cl[c].dt > 0.0) || !(b->cl[c].c_tilde > 0.0))
+            bfail(b, M1LYA_ERR_STATE, who + ": step parameters not set");
+    }
+}
+
+void capture_graphs(m1lya_batch* b)
+{
+    const bool prof = b->profiling;
+    b->profiling = false;
+    for (int p = 0; p < 2; ++p) {
+        if (b->gexec[p]) continue;
+        hipGraph_t g;
+        bchk(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        launch_step_kernels(b, p);
+        bchk(b, hipStreamEndCapture(b->stream, &g), "hipStreamEndCapture");
+        bchk(b, hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        (void)hipGraphDestroy(g);
+    }
+    b->profiling = prof;
+}
+
+void run_steps(m1lya_batch* b, int n_steps, double* history)
+{
+    require_ready(b);
+    sync_clouds(b);
+    if (!b->krange_valid) compute_krange(b);
+    if (history && n_steps > b->hist_cap) {
+        bfree(b, b->hist);
+        b->hist_cap = std::max(n_steps, 64);
+        b->hist = balloc<double>(b, (size_t)b->hist_cap * b->n * 8);
+    }
+    bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    set_ctrl(b, history ? b->hist : nullptr, history ? b->hist_cap : 0);
+    const bool graph = b->use_graph && !b->profiling;
+    if (graph) capture_graphs(b);
+    for (int s = 0; s < n_steps; ++s) {
+        if (graph)
+            bchk(b, hipGraphLaunch(b->gexec[b->parity], b->stream), "hipGraphLaunch");
+        else
+            launch_step_kernels(b, b->parity);
+        b->parity ^= 1;
+    }
+    if (history) {
+        bchk(b, hipMemcpyAsync(history, b->hist, (size_t)n_steps * b->n * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream),
+             "history download");
+    }
+    bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+void up_f32(m1lya_batch* b, float* dst, const double* src, size_t n, double scale)
+{
+    std::vector<float> h(n);
+    for (size_t t = 0; t < n; ++t) h[t] = (float)(src[t] * scale);
+    bchk(b, hipMemcpy(dst, h.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int* n_z, m1lya_batch** out)
+{
+    if (!out || !n_nu || !n_r || !n_z || n_clouds < 1) {
+        g_batch_error = "m1lya_batch_create: bad arguments";
+        return M1LYA_ERR_ARG;
+    }
+    *out = nullptr;
+    for (int c = 0; c < n_clouds; ++c)
+        if (n_nu[c] < 3 || n_nu[c] > kMaxBins || n_r[c] < 3 || n_z[c] < 3) {
+            g_batch_error = "m1lya_batch_create: cloud " + std::to_string(c) +
+                            " needs 3 <= N_nu <= 128 and N_r, N_Z >= 3";
+            return M1LYA_ERR_DIMS;
+        }
+    m1lya_batch* b = new m1lya_batch();
+    const int rc = bguard(b, [&] {
+        b->n = n_clouds;
+        bchk(b, hipGetDevice(&b->device), "hipGetDevice");
+        bchk(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        b->cl.resize(n_clouds);
+        b->geom_set.assign(n_clouds, 0);
+        b->grid_set.assign(n_clouds, 0);
+        for (int c = 0; c < n_clouds; ++c) {
+            Cloud& cl = b->cl[c];
+            std::memset(&cl, 0, sizeof(Cloud));
+            cl.nn = n_nu[c]; cl.nr = n_r[c]; cl.nz = n_z[c];
+            cl.R = cl.nn <= 64 ? 1 : 2;
+            const size_t ncell = (size_t)cl.nr * cl.nz;
+            cl.o3 = (long long)b->n3; b->n3 += ncell * cl.nn;
+            cl.oc = (long long)b->nc; b->nc += ncell;
+            cl.ofr = (long long)b->nfr; b->nfr += (size_t)(cl.nr + 1) * cl.nz;
+            cl.ofz = (long long)b->nfz; b->nfz += (size_t)cl.nr * (cl.nz + 1);
+            cl.obin = (long long)b->nbin; b->nbin += cl.nn;
+            cl.oivr = (long long)b->nivr; b->nivr += (size_t)(cl.nr + 1) * 5;
+            cl.oivz = (long long)b->nivz; b->nivz += (size_t)(cl.nz + 1) * 5;
+            cl.J0 = 1.0;
+            cl.bc_r_right = M1LYA_OUTFLOW; cl.bc_z_bot = M1LYA_REFLECT; cl.bc_z_top = M1LYA_OUTFLOW;
+        }
+        b->h_rec.assign(b->nc, CellRec{});
+        b->h_bin.assign(b->nbin, BinRec{});
+        Arrays& A = b->A;
+        for (int p = 0; p < 2; ++p) {
+            A.J[p] = balloc<float>(b, b->n3);
+            A.Hr[p] = balloc<float>(b, b->n3);
+            A.Hz[p] = balloc<float>(b, b->n3);
+        }
+        A.aF = balloc<float>(b, b->n3);
+        A.Hv = balloc<float>(b, b->n3);
+        A.phi = balloc<float>(b, b->n3);
+        b->d_rec = balloc<CellRec>(b, b->nc);
+        b->d_bin = balloc<BinRec>(b, b->nbin);
+        A.rec = b->d_rec;
+        A.bin = b->d_bin;
+        A.uc_r = balloc<float>(b, b->nfr);
+        A.uc_z = balloc<float>(b, b->nfz);
+        A.ivr = balloc<float>(b, b->nivr);
+        A.ivz = balloc<float>(b, b->nivz);
+        A.f_r = balloc<double>(b, b->nc);
+        A.f_z = balloc<double>(b, b->nc);
+        b->d_cl = balloc<Cloud>(b, n_clouds);
+        b->red = balloc<double>(b, (size_t)n_clouds * kPartStride);
+        b->krange = balloc<int>(b, (size_t)n_clouds * 2);
+        b->scalars = balloc<double>(b, (size_t)n_clouds * 8);
+        b->ctrl = balloc<Ctrl>(b, 1);
+        build_tiles(b);
+    });
+    if (rc != M1LYA_OK) {
+        m1lya_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return M1LYA_OK;
+}
+
+void m1lya_batch_destroy(m1lya_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int m1lya_batch_last_error(const m1lya_batch* b, char* buf, size_t len)
+{
+    if (!buf || len == 0) return M1LYA_ERR_ARG;
+    const std::string& s = b ? b->err : g_batch_error;
+    std::snprintf(buf, len, "%s", s.c_str());
+    return M1LYA_OK;
+}
+
+int m1lya_batch_count(const m1lya_batch* b) { return b ? b->n : 0; }
+
+int m1lya_batch_set_geometry(m1lya_batch* b, int cloud, const m1lya_geometry* g)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!g || !g->dV || !g->inv_dV || !g->A_r || !g->A_Z || !g->inv_r_cell || !g->rCoM || !g->ZCoM || !g->dr_L ||
+            !g->dr_R || !g->inv_dr_i_im1 || !g->inv_dr_ip1_i || !g->inv_dr_ip2_ip1 || !g->dZ_L || !g->dZ_R ||
+            !g->inv_dZ_j_jm1 || !g->inv_dZ_jp1_j || !g->inv_dZ_jp2_jp1)
+            bfail(b, M1LYA_ERR_ARG, "geometry has a NULL array");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        const Cloud& cl = b->cl[cloud];
+        const int nr = cl.nr, nz = cl.nz;
+        CellRec* rec = b->h_rec.data() + cl.oc;
+        for (int j = 0; j < nz; ++j)
+            for (int i = 0; i < nr; ++i) {
+                const size_t c = (size_t)j * nr + i;
+                CellRec& r = rec[c];
+                // A/ΔV of each cell face: the flux-divergence factors of src/AdvectJandH.jl:1319-1340
+                r.gRl = (float)(g->A_r[(size_t)j * (nr + 1) + i] * g->inv_dV[c]);
+                r.gRr = (float)(g->A_r[(size_t)j * (nr + 1) + i + 1] * g->inv_dV[c]);
+                r.gZl = (float)(g->A_Z[(size_t)j * nr + i] * g->inv_dV[c]);
+                r.gZr = (float)(g->A_Z[(size_t)(j + 1) * nr + i] * g->inv_dV[c]);
+                r.inv_r = (float)g->inv_r_cell[i];
+                r.dV = g->dV[c];
+                r.rcom = g->rCoM[i];
+                r.zcom = g->ZCoM[j];
+                // boundary faces of Compute_f_esc_Cylindrical (src/ConstantsAndFunctions.jl:937-953)
+                r.A_rout = (i == nr - 1) ? g->A_r[(size_t)j * (nr + 1) + nr] : 0.0;
+                r.A_ztop = (j == nz - 1) ? g->A_Z[(size_t)nz * nr + i] : 0.0;
+                r.A_zbot = (j == 0) ? g->A_Z[i] : 0.0;
+            }
+        push_records(b, cloud);
+        // one interval set per interface: face 0 inner boundary, face N outer boundary,
+        // face f in 1..N-1 the interior arrays at f-1 (src/AdvectJandH.jl:632-639, 803-812)
+        auto pack = [&](int n, const double* dL, const double* dR, const double* a, const double* bb,
+                        const double* cc, const double* inner, const double* outer, float* dst) {
+            std::vector<double> h((size_t)(n + 1) * 5);
+            for (int m = 0; m < 5; ++m) {
+                h[m] = inner[m];
+                h[(size_t)n * 5 + m] = outer[m];
+            }
+            for (int f = 1; f < n; ++f) {
+                h[(size_t)f * 5 + 0] = dL[f - 1];
+                h[(size_t)f * 5 + 1] = dR[f - 1];
+                h[(size_t)f * 5 + 2] = a[f - 1];
+                h[(size_t)f * 5 + 3] = bb[f - 1];
+                h[(size_t)f * 5 + 4] = cc[f - 1];
+            }
+            up_f32(b, dst, h.data(), h.size(), 1.0);
+        };
+        pack(nr, g->dr_L, g->dr_R, g->inv_dr_i_im1, g->inv_dr_ip1_i, g->inv_dr_ip2_ip1, g->dr_inner, g->dr_outer,
+             b->A.ivr + cl.oivr);
+        pack(nz, g->dZ_L, g->dZ_R, g->inv_dZ_j_jm1, g->inv_dZ_jp1_j, g->inv_dZ_jp2_jp1, g->dZ_inner, g->dZ_outer,
+             b->A.ivz + cl.oivz);
+        b->geom_set[cloud] = 1;
+        b->krange_valid = false;
+    });
+}
+
+int m1lya_batch_set_freq_grid(m1lya_batch* b, int cloud, const m1lya_freq_grid* g)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        Cloud& cl = b->cl[cloud];
+        if (!g || !g->nu || !g->nuC || !g->dnu || !g->inv_dnu || !g->inv_nuC)
+            bfail(b, M1LYA_ERR_ARG, "frequency grid has a NULL array");
+        if (g->kcore_min < 0 || g->kcore_max >= cl.nn || g->kcore_min > g->kcore_max)
+            bfail(b, M1LYA_ERR_ARG, "kcore range out of bounds");
+        if (g->flux_method != M1LYA_GLF && g->flux_method != M1LYA_HLL) bfail(b, M1LYA_ERR_ARG, "unknown flux method");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        BinRec* br = b->h_bin.data() + cl.obin;
+        for (int k = 0; k < cl.nn; ++k) {
+            br[k].nuC = g->nuC[k];
+            br[k].dnu = g->dnu[k];
+            br[k].inv_dnu = g->inv_dnu[k];
+            br[k].inv_nuC = g->inv_nuC[k];
+            br[k].half_R = g->nu[k + 1] - g->nuC[k];
+            br[k].half_L = g->nuC[k] - g->nu[k];
+        }
+        bchk(b, hipMemcpy(b->d_bin + cl.obin, br, cl.nn * sizeof(BinRec), hipMemcpyHostToDevice), "bins");
+        cl.kcore_min = g->kcore_min;
+        cl.kcore_max = g->kcore_max;
+        cl.recoil = g->recoil ? 1 : 0;
+        cl.hll = g->flux_method == M1LYA_HLL ? 1 : 0;
+        cl.alpha_floor = g->alpha_floor;
+        b->grid_set[cloud] = 1;
+        b->cl_dirty = true;
+        b->krange_valid = false;
+    });
+}
+
+int m1lya_batch_set_scale(m1lya_batch* b, int cloud, double J0)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!(J0 > 0.0) || !std::isfinite(J0)) bfail(b, M1LYA_ERR_ARG, "J0 must be finite and positive");
+        Cloud& cl = b->cl[cloud];
+        const double f = cl.J0 / J0;
+        if (f != 1.0) {
+            // keep the resident state's physical values
+            const size_t n = (size_t)cl.nn * cl.nr * cl.nz;
+            const int p = b->parity;
+            hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->A.J[p] + cl.o3,
+                               b->A.Hr[p] + cl.o3, b->A.Hz[p] + cl.o3, n, (float)f);
+            bchk(b, hipGetLastError(), "k_scale");
+            bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        }
+        cl.J0 = J0;
+        b->cl_dirty = true;
+        b->krange_valid = false;
+    });
+}
+
+int m1lya_batch_get_scale(const m1lya_batch* b, int cloud, double* J0)
+{
+    if (!b || !J0 || cloud < 0 || cloud >= b->n) return M1LYA_ERR_ARG;
+    *J0 = b->cl[cloud].J0;
+    return M1LYA_OK;
+}
+
+int m1lya_batch_upload(m1lya_batch* b, int cloud, int field, const double* host, size_t count)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!host) bfail(b, M1LYA_ERR_ARG, "host pointer is NULL");
+        Cloud& cl = b->cl[cloud];
+        const size_t ncell = (size_t)cl.nr * cl.nz, n3 = ncell * cl.nn;
+        const size_t nfr = (size_t)(cl.nr + 1) * cl.nz, nfz = (size_t)cl.nr * (cl.nz + 1);
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        Arrays& A = b->A;
+        const int p = b->parity;
+        auto need = [&](size_t n) {
+            if (count != n)
+                bfail(b, M1LYA_ERR_ARG,
+                      "field " + std::to_string(field) + ": count " + std::to_string(count) + " != " +
+                          std::to_string(n));
+        };
+        CellRec* rec = b->h_rec.data() + cl.oc;
+        auto cellfield = [&](double CellRec::*m) {
+            need(ncell);
+            for (size_t c = 0; c < ncell; ++c) rec[c].*m = host[c];
+            push_records(b, cloud);
+        };
+        const double inv_J0 = 1.0 / cl.J0;
+        switch (field) {
+        case M1LYA_J: need(n3); up_f32(b, A.J[p] + cl.o3, host, n3, inv_J0); b->krange_valid = false; break;
+        case M1LYA_H_R: need(n3); up_f32(b, A.Hr[p] + cl.o3, host, n3, inv_J0); break;
+        case M1LYA_H_Z: need(n3); up_f32(b, A.Hz[p] + cl.o3, host, n3, inv_J0); break;
+        case M1LYA_ALPHA_F: need(n3); up_f32(b, A.aF + cl.o3, host, n3, 1.0); break;
+        case M1LYA_HVOIGT: need(n3); up_f32(b, A.Hv + cl.o3, host, n3, 1.0); break;
+        case M1LYA_PHI_NU: need(n3); up_f32(b, A.phi + cl.o3, host, n3, 1.0); break;
+        case M1LYA_ALPHA_LYA: cellfield(&CellRec::alpha); break;
+        case M1LYA_ALPHA_C: cellfield(&CellRec::alpha_c); break;
+        case M1LYA_PD: cellfield(&CellRec::pd); break;
+        case M1LYA_JS: cellfield(&CellRec::js); break;
+        case M1LYA_T: cellfield(&CellRec::T); break;
+        case M1LYA_DNU_D: cellfield(&CellRec::dnuD); break;
+        case M1LYA_DUR_DR: cellfield(&CellRec::durdr); break;
+        case M1LYA_DUR_DZ: cellfield(&CellRec::durdz); break;
+        case M1LYA_DUZ_DR: cellfield(&CellRec::duzdr); break;
+        case M1LYA_DUZ_DZ: cellfield(&CellRec::duzdz); break;
+        case M1LYA_UR_OVER_R: cellfield(&CellRec::urr); break;
+        case M1LYA_UR_STAR: need(nfr); up_f32(b, A.uc_r + cl.ofr, host, nfr, kInvC); break;
+        case M1LYA_UZ_STAR: need(nfz); up_f32(b, A.uc_z + cl.ofz, host, nfz, kInvC); break;
+        default: bfail(b, M1LYA_ERR_ARG, "field " + std::to_string(field) + " is not held by a batch");
+        }
+        if (field >= M1LYA_DUR_DR && field <= M1LYA_UR_OVER_R) {
+            cl.has_vel = 1;
+            b->cl_dirty = true;
+        }
+    });
+}
+
+int m1lya_batch_download(m1lya_batch* b, int cloud, int field, double* host, size_t count)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!host) bfail(b, M1LYA_ERR_ARG, "host pointer is NULL");
+        const Cloud& cl = b->cl[cloud];
+        const size_t ncell = (size_t)cl.nr * cl.nz, n3 = ncell * cl.nn;
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        const int p = b->parity;
+        auto need = [&](size_t n) {
+            if (count != n) bfail(b, M1LYA_ERR_ARG, "field " + std::to_string(field) + ": wrong count");
+        };
+        auto down32 = [&](const float* src, size_t n, double scale) {
+            std::vector<float> h(n);
+            bchk(b, hipMemcpy(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost), "download");
+            for (size_t t = 0; t < n; ++t) host[t] = (double)h[t] * scale;
+        };
+        auto down64 = [&](const double* src, size_t n) {
+            bchk(b, hipMemcpy(host, src, n * sizeof(double), hipMemcpyDeviceToHost), "download");
+        };
+        switch (field) {
+        case M1LYA_J: need(n3); down32(b->A.J[p] + cl.o3, n3, cl.J0); break;
+        case M1LYA_H_R: need(n3); down32(b->A.Hr[p] + cl.o3, n3, cl.J0); break;
+        case M1LYA_H_Z: need(n3); down32(b->A.Hz[p] + cl.o3, n3, cl.J0); break;
+        case M1LYA_F_LYA_R: need(ncell); down64(b->A.f_r + cl.oc, ncell); break;
+        case M1LYA_F_LYA_Z: need(ncell); down64(b->A.f_z + cl.oc, ncell); break;
+        default: bfail(b, M1LYA_ERR_ARG, "field " + std::to_string(field) + " cannot be downloaded from a batch");
+        }
+    });
+}
+
+int m1lya_batch_set_step(m1lya_batch* b, int cloud, double c_tilde, double dt, double z_center, int bc_r_right,
+                         int bc_z_bot, int bc_z_top)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!(c_tilde > 0.0) || !(dt > 0.0)) bfail(b, M1LYA_ERR_ARG, "c_tilde and dt must be positive");
+        for (int bc : {bc_r_right, bc_z_bot, bc_z_top})
+            if (bc != M1LYA_REFLECT && bc != M1LYA_OUTFLOW) bfail(b, M1LYA_ERR_ARG, "invalid boundary condition");
+        Cloud& cl = b->cl[cloud];
+        cl.c_tilde = c_tilde;
+        cl.dt = dt;
+        cl.z_center = z_center;
+        cl.cdt = (float)(c_tilde * dt);
+        cl.cdtc = (float)(c_tilde * dt * kInvC);
+        cl.bc_r_right = bc_r_right;
+        cl.bc_z_bot = bc_z_bot;
+        cl.bc_z_top = bc_z_top;
+        b->cl_dirty = true;
+    });
+}
+
+int m1lya_batch_run_steps(m1lya_batch* b, int n_steps, double* history)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        if (n_steps < 0) bfail(b, M1LYA_ERR_ARG, "n_steps < 0");
+        run_steps(b, n_steps, history);
+    });
+}
+
+int m1lya_batch_scalars(m1lya_batch* b, double* out)
+{
+    return bguard(b, [&] {
+        if (!b || !out) bfail(b, M1LYA_ERR_ARG, "NULL argument");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        bchk(b, hipMemcpy(out, b->scalars, (size_t)b->n * 8 * sizeof(double), hipMemcpyDeviceToHost), "scalars");
+    });
+}
+
+int m1lya_batch_active_bins(m1lya_batch* b, int* out)
+{
+    return bguard(b, [&] {
+        if (!b || !out) bfail(b, M1LYA_ERR_ARG, "NULL argument");
+        require_ready(b);
+        if (!b->krange_valid) compute_krange(b);
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        bchk(b, hipMemcpy(out, b->krange, (size_t)b->n * 2 * sizeof(int), hipMemcpyDeviceToHost), "krange");
+    });
+}
+
+int m1lya_batch_set_tiles(m1lya_batch* b, int seg_len, int waves_per_block)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        if (seg_len < 1 || seg_len > 4096) bfail(b, M1LYA_ERR_ARG, "seg_len out of range [1, 4096]");
+        if (waves_per_block < 1 || waves_per_block > kMaxWaves) bfail(b, M1LYA_ERR_ARG, "waves_per_block must be 1..4");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        b->seg_len = seg_len;
+        b->wpb = waves_per_block;
+        build_tiles(b);
+    });
+}
+
+int m1lya_batch_get_tiles(const m1lya_batch* b, int* seg_len, int* waves_per_block)
+{
+    if (!b || !seg_len || !waves_per_block) return M1LYA_ERR_ARG;
+    *seg_len = b->seg_len;
+    *waves_per_block = b->wpb;
+    return M1LYA_OK;
+}
+
+int m1lya_batch_set_graph(m1lya_batch* b, int on)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        b->use_graph = on != 0;
+        if (!b->use_graph) drop_graphs(b);
+    });
+}
+
+// Times every tile shape of the candidate set on the live batch and keeps the
+// fastest.  State, active-bin ranges and buffer parity are restored, so the call
+// does not advance the simulation (the step scalars are those of the last trial).
+int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* waves_per_block, double* best_ms)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        if (trial_steps < 1) bfail(b, M1LYA_ERR_ARG, "trial_steps must be >= 1");
+        require_ready(b);
+        sync_clouds(b);
+        if (!b->krange_valid) compute_krange(b);
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        const int p0 = b->parity;
+        const size_t bytes = b->n3 * sizeof(float);
+        float* sJ = balloc<float>(b, b->n3);
+        float* sR = balloc<float>(b, b->n3);
+        float* sZ = balloc<float>(b, b->n3);
+        int* sK = balloc<int>(b, (size_t)b->n * 2);
+        auto copy = [&](void* dst, const void* src, size_t n) {
+            bchk(b, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream), "autotune copy");
+        };
+        copy(sJ, b->A.J[p0], bytes);
+        copy(sR, b->A.Hr[p0], bytes);
+        copy(sZ, b->A.Hz[p0], bytes);
+        copy(sK, b->krange, b->n * 2 * sizeof(int));
+        auto restore = [&] {
+            copy(b->A.J[p0], sJ, bytes);
+            copy(b->A.Hr[p0], sR, bytes);
+            copy(b->A.Hz[p0], sZ, bytes);
+            copy(b->krange, sK, b->n * 2 * sizeof(int));
+            b->parity = p0;
+        };
+        const bool prof = b->profiling;
+        b->profiling = false;
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        set_ctrl(b, nullptr, 0);
+        static const int seg_opts[] = {4, 8, 16, 32, 64};
+        static const int wpb_opts[] = {1, 2, 4};
+        int best_s = b->seg_len, best_w = b->wpb;
+        double best = 1e300;
+        hipEvent_t e0, e1;
+        bchk(b, hipEventCreate(&e0), "hipEventCreate");
+        bchk(b, hipEventCreate(&e1), "hipEventCreate");
+        for (int s : seg_opts)
+            for (int w : wpb_opts) {
+                b->seg_len = s;
+                b->wpb = w;
+                build_tiles(b);
+                sync_clouds(b);
+                restore();
+                launch_step_kernels(b, b->parity);   // warm-up
+                b->parity ^= 1;
+                bchk(b, hipEventRecord(e0, b->stream), "hipEventRecord");
+                for (int t = 0; t < trial_steps; ++t) {
+                    launch_step_kernels(b, b->parity);
+                    b->parity ^= 1;
+                }
+                bchk(b, hipEventRecord(e1, b->stream), "hipEventRecord");
+                bchk(b, hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                bchk(b, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                const double per = ms / trial_steps;
+                if (per < best) {
+                    best = per;
+                    best_s = s;
+                    best_w = w;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->seg_len = best_s;
+        b->wpb = best_w;
+        build_tiles(b);
+        sync_clouds(b);
+        restore();
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        bfree(b, sJ);
+        bfree(b, sR);
+        bfree(b, sZ);
+        bfree(b, sK);
+        b->profiling = prof;
+        if (seg_len) *seg_len = best_s;
+        if (waves_per_block) *waves_per_block = best_w;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+int m1lya_batch_synchronize(m1lya_batch* b)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    });
+}
+
+int m1lya_batch_set_profiling(m1lya_batch* b, int on)
+{
+    if (!b) return M1LYA_ERR_ARG;
+    b->profiling = on != 0;
+    return M1LYA_OK;
+}
+
+int m1lya_batch_kernel_stats_count(m1lya_batch* b) { return b ? (int)b->stats.size() : 0; }
+
+int m1lya_batch_kernel_stats(m1lya_batch* b, int index, char* name, size_t name_len, long* launches, double* total_ms)
+{
+    if (!b || index < 0 || index >= (int)b->stats.size()) return M1LYA_ERR_ARG;
+    auto it = b->stats.begin();
+    std::advance(it, index);
+    if (name && name_len) std::snprintf(name, name_len, "%s", it->first.c_str());
+    if (launches) *launches = it->second.launches;
+    if (total_ms) *total_ms = it->second.total_ms;
+    return M1LYA_OK;
+}
+
+int m1lya_batch_reset_kernel_stats(m1lya_batch* b)
+{
+    if (!b) return M1LYA_ERR_ARG;
+    b->stats.clear();
+    return M1LYA_OK;
+}
+
+}  // extern "C"

@@ -49,6 +49,23 @@ def test_bad_arguments_fail_loudly_without_a_device():
         _lib.check(lib.m1lya_create(1, 1, 1, 0, C.byref(h)))
 
 
+def test_batch_bad_arguments_fail_loudly_without_a_device():
+    lib = _lib.load()
+    h = C.c_void_p()
+    ints = lambda *v: (C.c_int * len(v))(*v)
+    # dimension checks run before any HIP call
+    assert lib.m1lya_batch_create(2, ints(64, 129), ints(8, 8), ints(8, 8), C.byref(h)) == _lib.ERR_DIMS
+    assert lib.m1lya_batch_create(1, ints(64), ints(2), ints(8), C.byref(h)) == _lib.ERR_DIMS
+    assert lib.m1lya_batch_create(0, ints(64), ints(8), ints(8), C.byref(h)) == _lib.ERR_ARG
+    buf = C.create_string_buffer(256)
+    lib.m1lya_batch_last_error(None, buf, 256)
+    assert buf.value
+    assert lib.m1lya_batch_run_steps(None, 1, None) == _lib.ERR_ARG
+    assert lib.m1lya_batch_set_tiles(None, 16, 4) == _lib.ERR_ARG
+    with pytest.raises(_lib.M1LyaError):
+        _lib.check_batch(lib.m1lya_batch_create(1, ints(2), ints(8), ints(8), C.byref(h)))
+
+
 def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     monkeypatch.setattr(_lib, "_lib", None)

@@ -1,0 +1,185 @@
+"""The mixed-precision batched step (m1lya_batch_* via M1LyaBatch) on the GPU.
+
+Reference: the FP64 path (M1LyaContext), which is bit-identical to the CPU
+oracle on these inputs (tests/test_gpu_parity.py), and the oracle itself on
+the small cases.  Bar: M_F and f_esc within 1e-4 relative of the FP64 result
+at every step compared, the stated target of the FP32 path; the frequency-
+integrated J energy within 1e-4.  Structural properties are exact: a cloud's
+state does not depend on which other clouds share its batch, on the tile
+shape, on graph replay, or on an autotuning pass.
+"""
+import numpy as np
+import pytest
+
+import lydion_amd as L
+from lydion_amd import m1lya as M
+from lydion_amd.batch import M1LyaBatch
+from lydion_amd.static_cloud import developed_state
+from oracle.oracle import OracleState
+from tests.helpers import bitdiff, small_cloud
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def lut():
+    return L.VoigtLUT()
+
+
+def _relerr(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _energy_err(cloud, J, Jref):
+    w = cloud.lya["dnu"][None, None, :] * cloud.geometry.dV[:, :, None]
+    return float(np.sum(np.abs(J - Jref) * w) / np.sum(np.abs(Jref) * w))
+
+
+def _fp64_history(cloud, nsteps):
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    h = M.run_steps(ctx, nsteps, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot)
+    return h, ctx
+
+
+def _check_history(hb, h64, what, start=0):
+    for s in range(start, h64.shape[0]):
+        for name, m in (("M_F", 0), ("f_esc", 4)):
+            e = _relerr(hb[s, m], h64[s, m])
+            assert e <= FP32_TOL, f"{what} step {s} {name}: fp32 {hb[s, m]:.9e} fp64 {h64[s, m]:.9e} rel {e:.2e}"
+
+
+def test_static_cloud_32x32x64_vs_oracle(lut):
+    """Plain static cloud (τ0 = 1e6, 64 bins) from its initial state: 60 steps against the oracle."""
+    cloud = L.static_cloud_32x32x64(lut=lut)
+    o = OracleState(cloud)
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(60)[:, 0, :]
+    ho = np.array([o.step() for _ in range(60)])
+    # the first steps carry the J floor (1e-30·J0 here, Jeps in FP64) in M_F's tiny values
+    _check_history(hb, ho, "static_cloud_32x32x64", start=5)
+    assert _energy_err(cloud, b.download(0, "J"), o["J"]) <= FP32_TOL
+
+
+@pytest.mark.parametrize("hll", [False, True])
+def test_cone_flow_vs_fp64(lut, hll):
+    """τ0 = 1e8, dust, cone source, Hubble outflow 200 km/s, 128 bins (two bins per lane)."""
+    cloud = developed_state(small_cloud(n_r=24, n_z=20, n_nu=128, tau0=1e8, tau_abs=1.0, theta_cone=np.pi / 4,
+                                        umax=200.0, lut=lut))
+    cloud.lya.hll = hll
+    h64, ctx = _fp64_history(cloud, 30)
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(30)[:, 0, :]
+    _check_history(hb, h64, f"cone_flow hll={hll}")
+    assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
+
+
+def test_outflow_bottom_and_reflecting_right(lut):
+    """Outflow Z boundary at the bottom and a reflecting outer r boundary."""
+    cloud = developed_state(small_cloud(n_r=20, n_z=18, n_nu=64, tau0=1e7, tau_abs=0.5, bc_z_bot=L.OUTFLOW,
+                                        lut=lut))
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    h64 = M.run_steps(ctx, 20, cloud.c_tilde, cloud.dt, cloud.Zc, bc_r_right=L.REFLECT, bc_z_bot=L.OUTFLOW)
+    b = M1LyaBatch([cloud])
+    b.set_step(0, cloud.c_tilde, cloud.dt, cloud.Zc, bc_r_right=L.REFLECT, bc_z_bot=L.OUTFLOW)
+    hb = b.run_steps(20)[:, 0, :]
+    _check_history(hb, h64, "outflow-bottom")
+
+
+def test_cone_dust_cloud_256x256x128_vs_fp64(lut):
+    """The full-size cloud: 8 steps from the developed state."""
+    cloud = developed_state(L.cone_dust_cloud_256x256x128(lut=lut))
+    h64, ctx = _fp64_history(cloud, 8)
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(8)[:, 0, :]
+    _check_history(hb, h64, "cone_dust_256x256x128")
+    assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
+
+
+def _mixed_clouds(lut):
+    return [
+        developed_state(small_cloud(n_r=16, n_z=12, n_nu=32, tau0=1e5, tau_abs=0.0, lut=lut), seed=1),
+        developed_state(small_cloud(n_r=9, n_z=21, n_nu=128, tau0=1e9, tau_abs=3.0, umax=800.0, lut=lut), seed=2),
+        developed_state(small_cloud(n_r=30, n_z=7, n_nu=64, tau0=1e7, tau_abs=10.0, theta_cone=np.pi / 4, lut=lut),
+                        seed=3),
+        developed_state(small_cloud(n_r=5, n_z=40, n_nu=100, tau0=1e6, tau_abs=1.0, bc_z_bot=L.OUTFLOW, lut=lut),
+                        seed=4),
+    ]
+
+
+def test_batch_equals_single_clouds(lut):
+    """A cloud's state and scalars do not depend on the other clouds of its batch."""
+    clouds = _mixed_clouds(lut)
+    big = M1LyaBatch(clouds)
+    hb = big.run_steps(6)
+    for idx, cl in enumerate(clouds):
+        one = M1LyaBatch([cl])
+        h1 = one.run_steps(6)
+        assert np.array_equal(hb[:, idx, :], h1[:, 0, :]), idx
+        for k in ("J", "H_r", "H_z", "f_r", "f_z"):
+            assert bitdiff(big.download(idx, k), one.download(0, k)) == 0, (idx, k)
+
+
+def test_mixed_batch_vs_fp64(lut):
+    """Every cloud of an uneven batch (bins 32..128, τ0 1e5..1e9, outflow, dust) against its FP64 run."""
+    clouds = _mixed_clouds(lut)
+    b = M1LyaBatch(clouds)
+    hb = b.run_steps(10)
+    for idx, cl in enumerate(clouds):
+        h64, _ = _fp64_history(cl, 10)
+        _check_history(hb[:, idx, :], h64, f"cloud {idx}")
+
+
+def test_tile_shape_does_not_change_state(lut):
+    clouds = _mixed_clouds(lut)[:2]
+    ref = M1LyaBatch(clouds)
+    ref.set_tiles(16, 4)
+    h_ref = ref.run_steps(4)
+    for tiles in [(4, 1), (7, 2), (64, 3)]:
+        b = M1LyaBatch(clouds)
+        b.set_tiles(*tiles)
+        h = b.run_steps(4)
+        for idx in range(len(clouds)):
+            assert bitdiff(b.download(idx, "J"), ref.download(idx, "J")) == 0, (tiles, idx)
+        # reductions are summed in a different order: agreement to rounding
+        assert np.max(np.abs(h[..., :5] - h_ref[..., :5]) / np.maximum(np.abs(h_ref[..., :5]), 1e-300)) < 1e-12
+
+
+def test_graph_replay_matches_eager(lut):
+    """hipGraph replays (one graph per buffer parity), with eager work and host
+    synchronisation between replays, reproduce the eager steps exactly."""
+    clouds = _mixed_clouds(lut)[:3]
+    eager = M1LyaBatch(clouds)
+    graph = M1LyaBatch(clouds)
+    graph.set_graph(True)
+    for rep in range(4):
+        he = eager.run_steps(3)
+        hg = graph.run_steps(3)
+        assert np.array_equal(he, hg), rep
+        graph.synchronize()
+        _ = graph.scalars()
+    for idx in range(len(clouds)):
+        assert bitdiff(graph.download(idx, "J"), eager.download(idx, "J")) == 0
+
+
+def test_autotune_restores_state(lut):
+    clouds = _mixed_clouds(lut)[:2]
+    a = M1LyaBatch(clouds)
+    b = M1LyaBatch(clouds)
+    s, w, ms = b.autotune(trial_steps=1)
+    assert (s, w) == b.tiles() and ms > 0
+    ha, hb = a.run_steps(3), b.run_steps(3)
+    for idx in range(len(clouds)):
+        assert bitdiff(a.download(idx, "J"), b.download(idx, "J")) == 0
+    assert np.max(np.abs(ha[..., :5] - hb[..., :5]) / np.maximum(np.abs(ha[..., :5]), 1e-300)) < 1e-12
+
+
+def test_scale_choice(lut):
+    """J0 only sets the FP32 exponent range: results agree to FP32 rounding for J0 x 1e6."""
+    cloud = developed_state(small_cloud(n_nu=64, tau0=1e7, lut=lut))
+    a = M1LyaBatch([cloud])
+    b = M1LyaBatch([cloud], scales=[a.scale(0) * 1e6])
+    ha, hb = a.run_steps(10), b.run_steps(10)
+    assert np.max(np.abs(ha[:, 0, 0] - hb[:, 0, 0]) / np.abs(ha[:, 0, 0])) < 1e-5
+    assert _energy_err(cloud, b.download(0, "J"), a.download(0, "J")) < 1e-5
