@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
                             st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
                             st.Hr[0] = wR[q][1]; st.Hr[1] = wR[q][2]; st.Hr[2] = wR[q][3]; st.Hr[3] = wR[q][0];
                             st.Hz[0] = wZ[q][1]; st.Hz[1] = wZ[q][2]; st.Hz[2] = wZ[q][3]; st.Hz[3] = wZ[q][0];
-                            iv = ivr + (size_t)(i + 1) * 5;
+                            iv = ivr + (size_t)(i + 1) * m1f::kIvStride;
                             aL = aF[((size_t)j * nr + max(i, 0)) * nn + k];
                             aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
                             u = ucr[(size_t)j * (nr + 1) + i + 1];
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
 #pragma unroll
                             for (int m = 0; m < 4; ++m)
                                 ld_z(J, Hr, Hz, nn, nr, nz, i, k, cells[m], bcb, bct, st.J[m], st.Hr[m], st.Hz[m]);
-                            iv = ivz + (size_t)g * 5;
+                            iv = ivz + (size_t)g * m1f::kIvStride;
                             aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
                             aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
                             u = ucz[(size_t)g * nr + i];
@@ -401,12 +401,7 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
                             Jn = m1f::fmaxf_(Jn - cdtcf * Ddu * Js, m1f::kJeps32);
                         }
                         // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
-                        const float Hm2 = Hrn * Hrn + Hzn * Hzn;
-                        if (Hm2 > Jn * Jn) {
-                            const float fac = Jn * m1f::rcp(m1f::fsqrt(Hm2));
-                            Hrn *= fac;
-                            Hzn *= fac;
-                        }
+                        m1f::causal_limit(Jn, Hrn, Hzn);
                     }
                     FL[q][0] = FR0; FL[q][1] = FR1; FL[q][2] = FR2;
                     kimR[q] = kimN;
@@ -1117,8 +1112,8 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
             cl.ofr = (long long)b->nfr; b->nfr += (size_t)(cl.nr + 1) * cl.nz;
             cl.ofz = (long long)b->nfz; b->nfz += (size_t)cl.nr * (cl.nz + 1);
             cl.obin = (long long)b->nbin; b->nbin += cl.nn;
-            cl.oivr = (long long)b->nivr; b->nivr += (size_t)(cl.nr + 1) * 5;
-            cl.oivz = (long long)b->nivz; b->nivz += (size_t)(cl.nz + 1) * 5;
+            cl.oivr = (long long)b->nivr; b->nivr += (size_t)(cl.nr + 1) * m1f::kIvStride;
+            cl.oivz = (long long)b->nivz; b->nivz += (size_t)(cl.nz + 1) * m1f::kIvStride;
             cl.J0 = 1.0;
             cl.bc_r_right = M1LYA_OUTFLOW; cl.bc_z_bot = M1LYA_REFLECT; cl.bc_z_top = M1LYA_OUTFLOW;
         }
@@ -1210,21 +1205,21 @@ int m1lya_batch_set_geometry(m1lya_batch* b, int cloud, const m1lya_geometry* g)
             }
         push_records(b, cloud);
         // one interval set per interface: face 0 inner boundary, face N outer boundary,
-        // face f in 1..N-1 the interior arrays at f-1 (src/AdvectJandH.jl:632-639, 803-812)
+        // face f in 1..N-1 the interior arrays at f-1 (src/AdvectJandH.jl:632-639, 803-812);
+        // the three inverse spacings are stored times ΔL or ΔR (half-cell units)
         auto pack = [&](int n, const double* dL, const double* dR, const double* a, const double* bb,
                         const double* cc, const double* inner, const double* outer, float* dst) {
-            std::vector<double> h((size_t)(n + 1) * 5);
-            for (int m = 0; m < 5; ++m) {
-                h[m] = inner[m];
-                h[(size_t)n * 5 + m] = outer[m];
-            }
-            for (int f = 1; f < n; ++f) {
-                h[(size_t)f * 5 + 0] = dL[f - 1];
-                h[(size_t)f * 5 + 1] = dR[f - 1];
-                h[(size_t)f * 5 + 2] = a[f - 1];
-                h[(size_t)f * 5 + 3] = bb[f - 1];
-                h[(size_t)f * 5 + 4] = cc[f - 1];
-            }
+            const int S = m1f::kIvStride;
+            std::vector<double> h((size_t)(n + 1) * S);
+            auto put = [&](int f, double vL, double vR, double ia, double ib, double ic) {
+                double* o = h.data() + (size_t)f * S;
+                o[0] = vL; o[1] = vR;
+                o[2] = vL * ia; o[3] = vL * ib;
+                o[4] = vR * ib; o[5] = vR * ic;
+            };
+            put(0, inner[0], inner[1], inner[2], inner[3], inner[4]);
+            put(n, outer[0], outer[1], outer[2], outer[3], outer[4]);
+            for (int f = 1; f < n; ++f) put(f, dL[f - 1], dR[f - 1], a[f - 1], bb[f - 1], cc[f - 1]);
             up_f32(b, dst, h.data(), h.size(), 1.0);
         };
         pack(nr, g->dr_L, g->dr_R, g->inv_dr_i_im1, g->inv_dr_ip1_i, g->inv_dr_ip2_ip1, g->dr_inner, g->dr_outer,

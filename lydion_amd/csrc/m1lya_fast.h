@@ -1,11 +1,19 @@
 // m1lya_fast.h -- FP32 device math of the mixed-precision M1 Lya step.
 //
 // Same formulas as the FP64 path (src/AdvectJandH.jl, src/M1Closure.jl), with
-// native reciprocal / sqrt and FMA contraction.  The state is normalised by a
-// per-cloud scale J0 (J' = J/J0, H' = H/J0), so FP32 covers it; the floor Jeps
-// becomes kJeps32 in those units.  Near free streaming the closure's
-// a = (1 - χ)/2 is evaluated in a cancellation-free form, and the cell-centred
-// closure (whose inputs are exact FP32 state values) switches to FP64 for f > 0.9.
+// native reciprocal / sqrt and FMA contraction, rearranged where FP32's range or
+// rounding would change the result:
+//  - the state is normalised by a per-cloud scale J0 (J' = J/J0, H' = H/J0) and the
+//    floor Jeps becomes kJeps32 in those units;
+//  - limited slopes are formed in units of the half-cell width (minmod(s·x, s·y) =
+//    s·minmod(x, y) for s > 0), so they never sit near the bottom of FP32's range
+//    the way per-cm slopes (~1e-22) and their products do;
+//  - the limiter tests signs, not the sign of a product that can underflow;
+//  - flux magnitudes are formed from H/J ratios, so |H|² never underflows;
+//  - the thick/thin blend uses 1 - ϵ = τ²/(num + τ²) and (1 - ϵ)/τ = τ/(num + τ²)
+//    (no cancellation, no division by τ);
+//  - the closure's a = (1 - χ)/2 is evaluated in a cancellation-free form, and the
+//    cell-centred closure switches to FP64 for f > 0.9.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,16 +27,22 @@ constexpr float kFmax32 = 1.0f - 1e-7f;
 constexpr float kThird = 1.0f / 3.0f;
 constexpr float kFreeStreamF = 0.9f;
 
+// Interval set of one interface in half-cell units: {ΔL, ΔR, ΔL/(x_i - x_{i-1}),
+// ΔL/(x_{i+1} - x_i), ΔR/(x_{i+1} - x_i), ΔR/(x_{i+2} - x_{i+1})}.
+constexpr int kIvStride = 6;
+
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fmaxf_(float a, float b) { return (b > a) ? b : a; }
 __device__ __forceinline__ float fminf_(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return (x > hi) ? hi : ((x < lo) ? lo : x); }
 
+// MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign.
 __device__ __forceinline__ float minmod(float A, float B)
 {
-    if (A * B <= 0.0f) return 0.0f;
-    return (fabsf(A) < fabsf(B)) ? A : B;
+    const bool same = (A > 0.0f && B > 0.0f) || (A < 0.0f && B < 0.0f);
+    return same ? ((fabsf(A) < fabsf(B)) ? A : B) : 0.0f;
 }
 
 // a = (1-χ)/2 and b = (3χ-1)/2 of the Levermore closure (src/M1Closure.jl:28-33), with
@@ -54,36 +68,36 @@ __device__ __forceinline__ void levermore_ab64(double f, double& a, double& b)
 
 struct Edd32 { float rr, zz, rz; };
 
-// Compute_D_Edd (src/M1Closure.jl:46-83), cell-centred: FP64 near free streaming.
+// Compute_D_Edd (src/M1Closure.jl:46-83), cell-centred, from the ratios H/J;
+// FP64 near free streaming.
 __device__ inline Edd32 D_edd(float Hr, float Hz, float J)
 {
     Edd32 D;
-    float Hm2 = Hr * Hr + Hz * Hz;
-    if (J <= kJeps32 || Hm2 <= 0.0f || Hm2 <= (kFmin32 * kFmin32) * J * J) {
-        D.rr = kThird; D.zz = kThird; D.rz = 0.0f;
-        return D;
-    }
-    float Hm = fsqrt(Hm2);
-    float f = Hm * rcp(J);
+    D.rr = kThird; D.zz = kThird; D.rz = 0.0f;
+    if (J <= kJeps32) return D;
+    const float iJ = rcp(J);
+    const float xr = Hr * iJ, xz = Hz * iJ;
+    const float f2 = xr * xr + xz * xz;
+    if (!(f2 > kFmin32 * kFmin32)) return D;
+    const float f = fsqrt(f2);
     if (f > kFreeStreamF) {
-        double hr = Hr, hz = Hz, j = J;
-        double hm = sqrt(hr * hr + hz * hz);
+        const double hr = Hr, hz = Hz, j = J;
+        const double hm = sqrt(hr * hr + hz * hz);
         double f64 = hm / j;
         f64 = f64 < 1e-10 ? 1e-10 : (f64 > 1.0 - 1e-10 ? 1.0 - 1e-10 : f64);
         double a, b;
         levermore_ab64(f64, a, b);
-        double nr = hr / hm, nz = hz / hm;
+        const double nr = hr / hm, nz = hz / hm;
         D.rr = (float)(a + b * nr * nr);
         D.zz = (float)(a + b * nz * nz);
         D.rz = (float)(b * nr * nz);
         return D;
     }
-    f = fclamp(f, kFmin32, kFmax32);
     float a, b;
-    levermore_ab(f, a, b);
-    float ih = rcp(Hm);
-    float nr = Hr * ih, nz = Hz * ih;
-    float bnr = b * nr;
+    levermore_ab(fclamp(f, kFmin32, kFmax32), a, b);
+    const float inf_ = rcp(f);
+    const float nr = xr * inf_, nz = xz * inf_;
+    const float bnr = b * nr;
     D.rr = a + bnr * nr;
     D.zz = a + b * nz * nz;
     D.rz = bnr * nz;
@@ -131,29 +145,24 @@ struct Face32 {
 };
 
 // ReconstructInterface -> closure -> GLF/HLL flux -> upwind advection flux
-// (src/AdvectJandH.jl:81-118, 155-204, 276-282, 359-473).
+// (src/AdvectJandH.jl:81-118, 155-204, 276-282, 359-473).  iv: kIvStride values.
 __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, const float* iv, float aF_L,
                                         float aF_R, float u_over_c)
 {
-    const float dL = iv[0], dR = iv[1], inv_i_im1 = iv[2], inv_ip1_i = iv[3], inv_ip2_ip1 = iv[4];
-    float tau_L = aF_L * dL, tau_R = aF_R * dR;
-    float iJ0 = rcp(fmaxf_(s.J[0], kJeps32)), iJ1 = rcp(fmaxf_(s.J[1], kJeps32));
-    float iJ2 = rcp(fmaxf_(s.J[2], kJeps32)), iJ3 = rcp(fmaxf_(s.J[3], kJeps32));
-    float fr0 = s.Hr[0] * iJ0, fr1 = s.Hr[1] * iJ1, fr2 = s.Hr[2] * iJ2, fr3 = s.Hr[3] * iJ3;
-    float fz0 = s.Hz[0] * iJ0, fz1 = s.Hz[1] * iJ1, fz2 = s.Hz[2] * iJ2, fz3 = s.Hz[3] * iJ3;
-    float dJc = (s.J[1] - s.J[0]) * inv_ip1_i;
-    float dJL = minmod((s.J[0] - s.J[3]) * inv_i_im1, dJc);
-    float dJR = minmod((s.J[2] - s.J[1]) * inv_ip2_ip1, dJc);
-    float dfrc = (fr1 - fr0) * inv_ip1_i;
-    float dfrL = minmod((fr0 - fr3) * inv_i_im1, dfrc);
-    float dfrR = minmod((fr2 - fr1) * inv_ip2_ip1, dfrc);
-    float dfzc = (fz1 - fz0) * inv_ip1_i;
-    float dfzL = minmod((fz0 - fz3) * inv_i_im1, dfzc);
-    float dfzR = minmod((fz2 - fz1) * inv_ip2_ip1, dfzc);
-    float J_L = fmaxf_(s.J[0] + dJL * dL, kJeps32);
-    float J_R = fmaxf_(s.J[1] - dJR * dR, kJeps32);
-    float fr_L = fr0 + dfrL * dL, fr_R = fr1 - dfrR * dR;
-    float fz_L = fz0 + dfzL * dL, fz_R = fz1 - dfzR * dR;
+    const float dL = iv[0], dR = iv[1], cL1 = iv[2], cL2 = iv[3], cR1 = iv[4], cR2 = iv[5];
+    const float tau_L = aF_L * dL, tau_R = aF_R * dR;
+    const float iJ0 = rcp(fmaxf_(s.J[0], kJeps32)), iJ1 = rcp(fmaxf_(s.J[1], kJeps32));
+    const float iJ2 = rcp(fmaxf_(s.J[2], kJeps32)), iJ3 = rcp(fmaxf_(s.J[3], kJeps32));
+    const float fr0 = s.Hr[0] * iJ0, fr1 = s.Hr[1] * iJ1, fr2 = s.Hr[2] * iJ2, fr3 = s.Hr[3] * iJ3;
+    const float fz0 = s.Hz[0] * iJ0, fz1 = s.Hz[1] * iJ1, fz2 = s.Hz[2] * iJ2, fz3 = s.Hz[3] * iJ3;
+    // left state: x_i + minmod(slope_{i-1/2}, slope_{i+1/2})·ΔL, slopes times ΔL
+    const float dJc = s.J[1] - s.J[0], dfrc = fr1 - fr0, dfzc = fz1 - fz0;
+    float J_L = fmaxf_(s.J[0] + minmod(cL1 * (s.J[0] - s.J[3]), cL2 * dJc), kJeps32);
+    float J_R = fmaxf_(s.J[1] - minmod(cR2 * (s.J[2] - s.J[1]), cR1 * dJc), kJeps32);
+    float fr_L = fr0 + minmod(cL1 * (fr0 - fr3), cL2 * dfrc);
+    float fr_R = fr1 - minmod(cR2 * (fr2 - fr1), cR1 * dfrc);
+    float fz_L = fz0 + minmod(cL1 * (fz0 - fz3), cL2 * dfzc);
+    float fz_R = fz1 - minmod(cR2 * (fz2 - fz1), cR1 * dfzc);
     float f_L = fsqrt(fr_L * fr_L + fz_L * fz_L);
     if (f_L > 1.0f) { float i = rcp(f_L); fr_L *= i; fz_L *= i; f_L = 1.0f; }
     float f_R = fsqrt(fr_R * fr_R + fz_R * fz_R);
@@ -161,9 +170,9 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
     float nr_L = 0.f, nz_L = 0.f, nr_R = 0.f, nz_R = 0.f;
     if (f_L > 0.0f) { float i = rcp(f_L); nr_L = fr_L * i; nz_L = fz_L * i; }
     if (f_R > 0.0f) { float i = rcp(f_R); nr_R = fr_R * i; nz_R = fz_R * i; }
-    float Hr_L = fr_L * J_L, Hr_R = fr_R * J_R, Hz_L = fz_L * J_L, Hz_R = fz_R * J_R;
-    Edd32 DL = D_edd_f_n(f_L, nr_L, nz_L);
-    Edd32 DR = D_edd_f_n(f_R, nr_R, nz_R);
+    const float Hr_L = fr_L * J_L, Hr_R = fr_R * J_R, Hz_L = fz_L * J_L, Hz_R = fz_R * J_R;
+    const Edd32 DL = D_edd_f_n(f_L, nr_L, nz_L);
+    const Edd32 DR = D_edd_f_n(f_R, nr_R, nz_R);
     float FJ_L, FJ_R, FHr_L, FHr_R, FHz_L, FHz_R;
     if (dir == 0) {
         FJ_L = Hr_L; FJ_R = Hr_R;
@@ -177,30 +186,30 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
     // ϵ = (0.2 + τ)/(0.2 + τ + τ²) and the Fick flux (1 - ϵ)·(-ΔJ/(3τ)), both from one
     // reciprocal: 1 - ϵ = τ²·den and (1 - ϵ)/τ = τ·den, so nothing is divided by τ and
     // 1 - ϵ does not cancel (in vacuum, τ = 0, both vanish exactly).
-    float tau = tau_L + tau_R;
-    float num = 0.2f + tau;
-    float den = rcp(num + tau * tau);
-    float eps = num * den;
-    float ome = tau * tau * den;
-    float F_J_diff_ome = -kThird * (s.J[1] - s.J[0]) * (tau * den);
+    const float tau = tau_L + tau_R;
+    const float num = 0.2f + tau;
+    const float den = rcp(num + tau * tau);
+    const float eps = num * den;
+    const float ome = tau * tau * den;
+    const float F_J_diff_ome = -kThird * dJc * (tau * den);
     float FJ, FHr, FHz;
     if (hll) {
         float l1L, l3L, l1R, l3R;
         hll_eigen(f_L, dir == 0 ? nr_L : nz_L, l1L, l3L);
         hll_eigen(f_R, dir == 0 ? nr_R : nz_R, l1R, l3R);
-        float S_R = fmaxf_(fmaxf_(l3L, l3R), 0.0f), S_L = fminf_(fminf_(l1L, l1R), 0.0f);
-        float idS = rcp(fmaxf_(S_R - S_L, 1e-30f));
-        float SRSL = S_R * S_L;
+        const float S_R = fmaxf_(fmaxf_(l3L, l3R), 0.0f), S_L = fminf_(fminf_(l1L, l1R), 0.0f);
+        const float idS = rcp(fmaxf_(S_R - S_L, 1e-30f));
+        const float SRSL = S_R * S_L;
         FJ = eps * ((S_R * FJ_L - S_L * FJ_R + SRSL * (J_R - J_L)) * idS) + F_J_diff_ome;
         FHr = eps * ((S_R * FHr_L - S_L * FHr_R + SRSL * (Hr_R - Hr_L)) * idS) + 0.5f * ome * (FHr_L + FHr_R);
         FHz = eps * ((S_R * FHz_L - S_L * FHz_R + SRSL * (Hz_R - Hz_L)) * idS) + 0.5f * ome * (FHz_L + FHz_R);
     } else {
         FJ = eps * (0.5f * (FJ_L + FJ_R) - 0.5f * (J_R - J_L)) + F_J_diff_ome;
-        float he = 0.5f * eps;
+        const float he = 0.5f * eps;
         FHr = 0.5f * (FHr_L + FHr_R) - he * (Hr_R - Hr_L);
         FHz = 0.5f * (FHz_L + FHz_R) - he * (Hz_R - Hz_L);
     }
-    bool up = (u_over_c < 0.0f);
+    const bool up = (u_over_c < 0.0f);
     Face32 r;
     r.FJ = FJ + u_over_c * (up ? J_R : J_L);
     r.FHr = FHr + u_over_c * (up ? Hr_R : Hr_L);
@@ -211,14 +220,29 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
     return r;
 }
 
+// Compute_K_ϕϕ_InnerAxis (src/AdvectJandH.jl:499-522)
 __device__ inline float kphi_inner_axis(float J_axis, float Hz_axis)
 {
     J_axis = fmaxf_(J_axis, kJeps32);
-    float fz = fclamp(Hz_axis * rcp(J_axis), -1.0f, 1.0f);
-    float f = fabsf(fz);
-    float nz = (f > 0.0f) ? (fz > 0.0f ? 1.0f : -1.0f) : 0.0f;
-    Edd32 D = D_edd_f_n(f, 0.0f, nz);
+    const float fz = fclamp(Hz_axis * rcp(J_axis), -1.0f, 1.0f);
+    const float f = fabsf(fz);
+    const float nz = (f > 0.0f) ? (fz > 0.0f ? 1.0f : -1.0f) : 0.0f;
+    const Edd32 D = D_edd_f_n(f, 0.0f, nz);
     return fclamp(1.0f - D.rr - D.zz, 0.0f, 1.0f) * J_axis;
+}
+
+// EnforceCausalReducedFlux (src/AdvectJandH.jl:478-493) in ratio form: scales
+// (Hr, Hz) to |H| = J when |H| > J, without forming |H|² (which can underflow).
+__device__ __forceinline__ void causal_limit(float J, float& Hr, float& Hz)
+{
+    const float iJ = rcp(J);
+    const float xr = Hr * iJ, xz = Hz * iJ;
+    const float m2 = xr * xr + xz * xz;
+    if (m2 > 1.0f) {
+        const float fac = frsq(m2);
+        Hr *= fac;
+        Hz *= fac;
+    }
 }
 
 }  // namespace m1f

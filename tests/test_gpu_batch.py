@@ -43,22 +43,29 @@ def _fp64_history(cloud, nsteps):
     return h, ctx
 
 
-def _check_history(hb, h64, what, start=0):
+def _check_history(hb, h64, what, start=0, fesc_min=0.0):
+    """M_F at every step from `start`; f_esc once the FP64 value exceeds fesc_min."""
     for s in range(start, h64.shape[0]):
         for name, m in (("M_F", 0), ("f_esc", 4)):
+            if m == 4 and not h64[s, m] > fesc_min:
+                continue
             e = _relerr(hb[s, m], h64[s, m])
             assert e <= FP32_TOL, f"{what} step {s} {name}: fp32 {hb[s, m]:.9e} fp64 {h64[s, m]:.9e} rel {e:.2e}"
 
 
 def test_static_cloud_32x32x64_vs_oracle(lut):
-    """Plain static cloud (τ0 = 1e6, 64 bins) from its initial state: 60 steps against the oracle."""
+    """Plain static cloud (τ0 = 1e6, 64 bins) from its initial state, 200 steps against
+    the oracle.  Until the radiation front reaches the box edge (about step 80), the
+    escaping flux is at the level of the J floor (1e-30·J0 in FP32, Jeps = 1e-60 in
+    FP64), so f_esc is compared from 1e-12 upwards; M_F from step 5."""
     cloud = L.static_cloud_32x32x64(lut=lut)
     o = OracleState(cloud)
     b = M1LyaBatch([cloud])
-    hb = b.run_steps(60)[:, 0, :]
-    ho = np.array([o.step() for _ in range(60)])
-    # the first steps carry the J floor (1e-30·J0 here, Jeps in FP64) in M_F's tiny values
-    _check_history(hb, ho, "static_cloud_32x32x64", start=5)
+    hb = b.run_steps(200)[:, 0, :]
+    ho = np.array([o.step() for _ in range(200)])
+    assert ho[-1, 4] > 1e-6   # the front has reached the boundary
+    _check_history(hb, ho, "static_cloud_32x32x64", start=5, fesc_min=1e-12)
+    assert np.array_equal(hb[:, 5:7], ho[:, 5:7])   # same active bins every step
     assert _energy_err(cloud, b.download(0, "J"), o["J"]) <= FP32_TOL
 
 
