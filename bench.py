@@ -1,17 +1,25 @@
 """Benchmark of the multi-frequency M1 Lyα step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
 
 Workload: the 256×256 r-Z cloud with 128 frequency bins (τ0 = 1e8, T = 1e4 K,
 polar cone of half-angle π/4, τ_abs = 1, point source of radius 0.5 cells) in a
 seeded, developed radiation field with every bin active.  A step is one pass
 of the RT-only time-loop body (advection, emission/absorption, frequency
-diffusion solve, M_F / f_esc reductions) over the whole cloud, in FP64.
+diffusion solve, M_F / f_esc reductions) over the whole cloud.
 
-Metric: cell-frequency updates per second = N_r·N_Z·N_ν·K / wall time, with the
-state resident in HBM.  The JSON line also carries the roofline of the dominant
-kernel (algorithmic bytes per launch / its mean duration from HIP events on
-the library's stream during the timed region) and the CPU oracle (the FP64
+Default (--precision fp32): the mixed-precision batched step (FP32 state, flux
+sweep and closure; FP64 emission, frequency factors and tridiagonal solve;
+compensated reductions) for a batch of this one cloud, tile shape chosen by the
+library's autotuner before the timed region.  Beside it the FP64 plain step
+(bit-identical to the reference restatement) is timed on the same state, and
+the accuracy block states the FP32 step's error in M_F, f_esc and the
+frequency-integrated J against the FP64 step after the same number of steps.
+
+Metric: cell-frequency updates per second = N_r·N_Z·N_ν·K / wall time, state
+resident in HBM.  The JSON line carries the roofline of the dominant kernel
+(algorithmic bytes per launch / its mean duration from HIP events on the
+library's stream during the timed region) and the CPU oracle (the FP64
 restatement of the reference) timed on the host cores on a bounded sample.
 """
 from __future__ import annotations
@@ -29,11 +37,16 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per cell-frequency update of each kernel (FP64 words):
-#   advect: reads J, H_r, H_Z, α_F, writes J, H_r, H_Z               -> 7 words
-#   column: reads J, H_r, H_Z, ϕ_ν, Hvoigt, α_F, writes J, H_r, H_Z    -> 9 words
-ALGO_BYTES = {"advect": 7 * 8, "column": 9 * 8}
-STEP_ALGO_BYTES = 72   # one fused step: J, H_r, H_Z read+write, α_F, Hvoigt, ϕ_ν read (FP64)
+# Algorithmic HBM bytes per cell-frequency update of each kernel:
+#   advect (FP64): reads J, H_r, H_Z, α_F, writes J, H_r, H_Z                     -> 7 x 8 B
+#   column (FP64): reads J, H_r, H_Z, ϕ_ν, Hvoigt, α_F, writes J, H_r, H_Z          -> 9 x 8 B
+#   k_step (FP32 state, one fused pass): reads J, H_r, H_Z, α_F, Hvoigt, ϕ_ν,
+#                                         writes J, H_r, H_Z                       -> 9 x 4 B
+ALGO_BYTES = {"advect": 7 * 8, "column": 9 * 8, "k_step": 9 * 4}
+STEP_ALGO_BYTES = {"fp64": 72, "fp32": 36}
+METRIC = "Lyα M1 cell-freq updates/s + %HBM peak; FP32/batched vs FP64 at M_F err≤1e-4"
+WORKLOAD = ("cone cloud 256x256 r-Z, 128 bins, tau0=1e8, T=1e4 K, cone half-angle pi/4, tau_abs=1, "
+            "point source of radius 0.5 cells")
 
 
 def parse():
@@ -41,9 +54,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     p.add_argument("--n-r", type=int, default=256)
     p.add_argument("--n-z", type=int, default=256)
     p.add_argument("--n-nu", type=int, default=128)
+    p.add_argument("--accuracy-steps", type=int, default=8)
+    p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU-oracle sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--copy-calibration", action="store_true",
@@ -80,7 +96,7 @@ def cpu_baseline(cloud_factory, seconds: float):
             break
     upd = cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
     return {"value": upd * n / el, "unit": "cell-freq updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the same {cloud.lya.n_r}x{cloud.lya.n_z}x{cloud.lya.n_nu} workload "
+            "sample": f"{n} FP64 steps of the same {cloud.lya.n_r}x{cloud.lya.n_z}x{cloud.lya.n_nu} workload "
                       f"({el:.1f} s, OMP_NUM_THREADS={threads})"}
 
 
@@ -91,6 +107,90 @@ def copy_calibration():
     M.selftest_math(3, np.ones(n))
 
 
+def run_fp64(cloud, steps, warmup):
+    """Timed FP64 plain steps; returns throughput, ms/step and per-kernel event totals."""
+    from lydion_amd import m1lya as M
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    args = (cloud.c_tilde, cloud.dt, cloud.Zc)
+    kw = dict(bc_z_bot=cloud.bc_z_bot, history=False)
+    M.run_steps(ctx, warmup, *args, **kw)
+    ctx.synchronize()
+    ctx.reset_kernel_stats()
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    M.run_steps(ctx, steps, *args, **kw)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    if not np.all(np.isfinite(ctx.download("J"))):
+        raise RuntimeError("non-finite J after the FP64 steps")
+    ctx.close()
+    return wall, stats
+
+
+def run_fp32(cloud, steps, warmup, autotune):
+    """Timed batched FP32 steps (a batch of this cloud).  The timed region launches the
+    kernels eagerly with HIP events around each one (no host synchronisation); a second,
+    untimed-for-the-headline pass replays the captured hipGraph."""
+    from lydion_amd.batch import M1LyaBatch
+    b = M1LyaBatch([cloud])
+    tiles = b.autotune(trial_steps=3) if autotune else (*b.tiles(), None)
+    b.run_steps(warmup, history=False)
+    b.reset_kernel_stats()
+    b.set_profiling(True)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run_steps(steps, history=False)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    stats = b.kernel_stats()
+    b.set_profiling(False)
+    b.set_graph(True)
+    b.run_steps(2, history=False)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.run_steps(steps, history=False)
+    b.synchronize()
+    graph_wall = time.perf_counter() - t1
+    if not np.all(np.isfinite(b.download(0, "J"))):
+        raise RuntimeError("non-finite J after the FP32 steps")
+    b.close()
+    return wall, stats, graph_wall, tiles
+
+
+def accuracy(cloud, nsteps):
+    """FP32 batched vs FP64 plain from the same state after nsteps steps."""
+    from lydion_amd import m1lya as M
+    from lydion_amd.batch import M1LyaBatch
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    h64 = M.run_steps(ctx, nsteps, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot)
+    b = M1LyaBatch([cloud])
+    h32 = b.run_steps(nsteps)[:, 0, :]
+    rel = lambda a, r: float(np.max(np.abs(a - r) / np.maximum(np.abs(r), 1e-300)))
+    w = cloud.lya["dnu"][None, None, :] * cloud.geometry.dV[:, :, None]
+    J64, J32 = ctx.download("J"), b.download(0, "J")
+    out = {"steps": nsteps, "M_F_max_rel_err": rel(h32[:, 0], h64[:, 0]),
+           "f_esc_max_rel_err": rel(h32[:, 4], h64[:, 4]),
+           "J_energy_rel_err": float(np.sum(np.abs(J32 - J64) * w) / np.sum(np.abs(J64) * w)),
+           "M_F_fp64_last": float(h64[-1, 0]), "f_esc_fp64_last": float(h64[-1, 4])}
+    ctx.close()
+    b.close()
+    return out
+
+
+def roofline(stats, dom, upd_per_step, steps, workload):
+    launches, total_ms = stats[dom]
+    avg_s = total_ms * 1e-3 / launches
+    achieved = ALGO_BYTES[dom] * upd_per_step / avg_s / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(dom, workload),
+            "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * upd_per_step,
+            "avg_launch_ms": round(1e3 * avg_s, 4),
+            "kernels_ms_per_step": {k: round(v[1] / steps, 4) for k, v in sorted(stats.items())}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -99,71 +199,66 @@ def main():
         sys.exit(2)
 
     import lydion_amd as L
-    from lydion_amd import m1lya as M
     from lydion_amd.static_cloud import developed_state
 
     def factory():
         return L.cone_dust_cloud_256x256x128(n_r=args.n_r, n_z=args.n_z, n_nu=args.n_nu)
 
     cloud = developed_state(factory())
-    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
     lya = cloud.lya
     upd_per_step = lya.n_nu * lya.n_r * lya.n_z
-    step_args = (cloud.c_tilde, cloud.dt, cloud.Zc)
-    kw = dict(bc_z_bot=cloud.bc_z_bot, history=False)
+    grid = f"{lya.n_r}x{lya.n_z}x{lya.n_nu}"
 
-    M.run_steps(ctx, args.warmup, *step_args, **kw)
-    ctx.synchronize()
-    ctx.reset_kernel_stats()
-    ctx.set_profiling(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    M.run_steps(ctx, args.steps, *step_args, **kw)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    stats = ctx.kernel_stats()
-    ctx.set_profiling(False)
-    J = ctx.download("J")
-    if not np.all(np.isfinite(J)):
-        raise RuntimeError("non-finite J after the timed steps")
+    fp64_steps = max(10, args.steps // 3) if args.precision == "fp32" else args.steps
+    wall64, stats64 = run_fp64(cloud, fp64_steps, args.warmup)
+    fp64 = {"value": upd_per_step * fp64_steps / wall64, "ms_per_step": 1e3 * wall64 / fp64_steps,
+            "steps": fp64_steps,
+            "roofline": roofline(stats64, max((k for k in stats64 if k in ALGO_BYTES), key=lambda k: stats64[k][1]),
+                                 upd_per_step, fp64_steps, f"cone_cloud_{grid}_fp64")}
+
+    if args.precision == "fp32":
+        wall, stats, graph_wall, tiles = run_fp32(cloud, args.steps, args.warmup, not args.no_autotune)
+        value = upd_per_step * args.steps / wall
+        dom = "k_step"
+        workload = f"cone_cloud_{grid}_fp32"
+        extra = {"fp64_plain": fp64, "speedup_vs_fp64": value / fp64["value"],
+                 "graph_ms_per_step": 1e3 * graph_wall / args.steps,
+                 "tiles": {"seg_len": tiles[0], "waves_per_block": tiles[1]},
+                 "accuracy": accuracy(cloud, args.accuracy_steps)}
+        dtype = "f32+f64"
+    else:
+        wall, stats, value = wall64, stats64, fp64["value"]
+        dom = max((k for k in stats if k in ALGO_BYTES), key=lambda k: stats[k][1])
+        workload = f"cone_cloud_{grid}_fp64"
+        extra = {}
+        dtype = "f64"
+    steps = args.steps if args.precision == "fp32" else fp64_steps
     if args.copy_calibration:
         copy_calibration()
-
-    value = upd_per_step * args.steps / wall
-    ms_per_step = 1e3 * wall / args.steps
-    dom = max((k for k in stats if k in ALGO_BYTES), key=lambda k: stats[k][1])
-    launches, total_ms = stats[dom]
-    avg_s = total_ms * 1e-3 / launches
-    achieved = ALGO_BYTES[dom] * upd_per_step / avg_s / 1e9
-    workload = f"cone_cloud_{lya.n_r}x{lya.n_z}x{lya.n_nu}_fp64"
-    traffic = load_traffic(dom, workload)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * upd_per_step,
-                "avg_launch_ms": round(1e3 * avg_s, 4),
-                "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in sorted(stats.items())}}
-    step_gbs = STEP_ALGO_BYTES * value / 1e9
+    step_gbs = STEP_ALGO_BYTES[args.precision] * value / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline(factory, args.cpu_seconds)
     line = {
-        "metric": "Lyα M1 cell-freq updates/s + %HBM peak; FP32/batched vs FP64 at M_F err≤1e-4",
+        "metric": METRIC,
         "value": value,
         "unit": "cell-freq updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (seeded developed radiation field, every bin active)",
-        "config": {"workload": "static cloud 256x256 r-Z, 128 bins, tau0=1e8, T=1e4 K, cone pi/4, tau_abs=1, "
-                               "point source; FP64 plain step",
-                   "grid": f"{lya.n_r}x{lya.n_z}", "n_nu": lya.n_nu, "precision": "fp64",
-                   "step_hbm_frac_72B": round(step_gbs / HBM_PEAK_GBS, 4)},
-        "roofline": roofline,
+        "config": {"workload": WORKLOAD + ("; FP32 batched step (batch of 1)" if args.precision == "fp32"
+                                           else "; FP64 plain step"),
+                   "grid": f"{lya.n_r}x{lya.n_z}", "n_nu": lya.n_nu, "precision": args.precision,
+                   "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "step_algo_bytes_per_update": STEP_ALGO_BYTES[args.precision]},
+        "roofline": roofline(stats, dom, upd_per_step, steps, workload),
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

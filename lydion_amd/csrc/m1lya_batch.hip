@@ -799,7 +799,7 @@ struct m1lya_batch {
     bool krange_valid = false;
 
     // tile shape and the work / partial-row tables it implies
-    int seg_len = 16, wpb = 4;
+    int seg_len = 32, wpb = 4;
     std::vector<Seg> segs;
     Seg* d_segs = nullptr;
     size_t segs_cap = 0;
@@ -821,6 +821,12 @@ struct m1lya_batch {
     std::string err;
     bool profiling = false;
     std::map<std::string, KStatB> stats;
+    struct Pending {
+        const char* name;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Pending> pending;         // event pairs not yet read
+    std::vector<hipEvent_t> event_pool;
 };
 
 namespace {
@@ -946,7 +952,20 @@ void set_ctrl(m1lya_batch* b, double* hist, int cap)
     bchk(b, hipMemcpy(b->ctrl, &b->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice), "ctrl");
 }
 
-// Brackets one launch with events when profiling (and waits for it).
+hipEvent_t pool_event(m1lya_batch* b)
+{
+    if (!b->event_pool.empty()) {
+        hipEvent_t e = b->event_pool.back();
+        b->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    bchk(b, hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// Brackets one launch with events on the batch stream when profiling; the pair is
+// read later (collect_stats), so profiling adds no host synchronisation.
 struct BProf {
     m1lya_batch* b;
     const char* name;
@@ -954,8 +973,8 @@ struct BProf {
     BProf(m1lya_batch* b_, const char* n) : b(b_), name(n)
     {
         if (b->profiling) {
-            bchk(b, hipEventCreate(&e0), "hipEventCreate");
-            bchk(b, hipEventCreate(&e1), "hipEventCreate");
+            e0 = pool_event(b);
+            e1 = pool_event(b);
             bchk(b, hipEventRecord(e0, b->stream), "hipEventRecord");
         }
     }
@@ -964,17 +983,26 @@ struct BProf {
         bchk(b, hipGetLastError(), name);
         if (b->profiling) {
             bchk(b, hipEventRecord(e1, b->stream), "hipEventRecord");
-            bchk(b, hipEventSynchronize(e1), "hipEventSynchronize");
-            float ms = 0.f;
-            bchk(b, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-            KStatB& s = b->stats[name];
-            s.launches += 1;
-            s.total_ms += ms;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            b->pending.push_back(m1lya_batch::Pending{name, e0, e1});
         }
     }
 };
+
+void collect_stats(m1lya_batch* b)
+{
+    if (b->pending.empty()) return;
+    bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    for (auto& p : b->pending) {
+        float ms = 0.f;
+        bchk(b, hipEventElapsedTime(&ms, p.e0, p.e1), "hipEventElapsedTime");
+        KStatB& st = b->stats[p.name];
+        st.launches += 1;
+        st.total_ms += ms;
+        b->event_pool.push_back(p.e0);
+        b->event_pool.push_back(p.e1);
+    }
+    b->pending.clear();
+}
 
 void launch_step_kernels(m1lya_batch* b, int parity)
 {
@@ -1158,6 +1186,11 @@ void m1lya_batch_destroy(m1lya_batch* b)
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
+    for (auto& p : b->pending) {
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1559,11 +1592,19 @@ int m1lya_batch_set_profiling(m1lya_batch* b, int on)
     return M1LYA_OK;
 }
 
-int m1lya_batch_kernel_stats_count(m1lya_batch* b) { return b ? (int)b->stats.size() : 0; }
+int m1lya_batch_kernel_stats_count(m1lya_batch* b)
+{
+    if (!b) return 0;
+    const int rc = bguard(b, [&] { collect_stats(b); });
+    return rc == M1LYA_OK ? (int)b->stats.size() : rc;
+}
 
 int m1lya_batch_kernel_stats(m1lya_batch* b, int index, char* name, size_t name_len, long* launches, double* total_ms)
 {
-    if (!b || index < 0 || index >= (int)b->stats.size()) return M1LYA_ERR_ARG;
+    if (!b) return M1LYA_ERR_ARG;
+    const int rc = bguard(b, [&] { collect_stats(b); });
+    if (rc != M1LYA_OK) return rc;
+    if (index < 0 || index >= (int)b->stats.size()) return M1LYA_ERR_ARG;
     auto it = b->stats.begin();
     std::advance(it, index);
     if (name && name_len) std::snprintf(name, name_len, "%s", it->first.c_str());
@@ -1575,8 +1616,10 @@ int m1lya_batch_kernel_stats(m1lya_batch* b, int index, char* name, size_t name_
 int m1lya_batch_reset_kernel_stats(m1lya_batch* b)
 {
     if (!b) return M1LYA_ERR_ARG;
-    b->stats.clear();
-    return M1LYA_OK;
+    return bguard(b, [&] {
+        collect_stats(b);
+        b->stats.clear();
+    });
 }
 
 }  // extern "C"
