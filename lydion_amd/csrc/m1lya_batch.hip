@@ -81,7 +81,7 @@ struct Cloud {
 };
 
 struct Seg {
-    int cloud, j, i0, i1;          // cells [i0, i1) of row j; cloud < 0: idle wavefront
+    int cloud, j, i0, i1;          // cells [i0, i1) of row j (j >= N_Z: a ghost row of the block)
 };
 
 struct Ctrl {
@@ -121,7 +121,17 @@ __device__ __forceinline__ double wave_sum64(double v)
     return v;
 }
 
-// 1/x: hardware estimate plus two Newton steps.
+// 1/x from the hardware estimate (relative error ~2^-24): drcp0 as is, for values that
+// only set a relaxation rate; drcp1 after one Newton step (~2e-15), for pivots and
+// quantities that set an equilibrium; drcp after two (a few ulp).
+__device__ __forceinline__ double drcp0(double x) { return __builtin_amdgcn_rcp(x); }
+
+__device__ __forceinline__ double drcp1(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 __device__ __forceinline__ double drcp(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
@@ -208,35 +218,6 @@ __device__ inline void D_edd64(double Hr, double Hz, double J, double& Drr, doub
     Drz = b * nr * nz;
 }
 
-// 1 - ϕ and χ of the exponential fitting, both sides of one P (src/FreqFactors.jl:65-118):
-// oL = exp(-P), xL = (1 - oL)/P, oR = exp(P), xR = (1 - oR)/P; P is already clamped to ±40.
-__device__ inline void fit_both(double P, double& oL, double& xL, double& oR, double& xR)
-{
-    if (fabs(P) < kPmin) {
-        const double h = 0.5 * (P * P);
-        oL = 1.0 - P + h; xL = 1.0 - 0.5 * P;
-        oR = 1.0 + P + h; xR = -1.0 - 0.5 * P;
-        return;
-    }
-    const double e = exp(-P);
-    const double iP = drcp(P);
-    oL = e;
-    xL = (1.0 - e) * iP;
-    oR = drcp(e);
-    xR = (1.0 - oR) * iP;
-}
-
-__device__ inline void fit_right(double P, double& oR, double& xR)
-{
-    if (fabs(P) < kPmin) {
-        oR = 1.0 + P + 0.5 * (P * P);
-        xR = -1.0 - 0.5 * P;
-        return;
-    }
-    oR = exp(P);
-    xR = (1.0 - oR) * drcp(P);
-}
-
 // Bins of one lane: R consecutive bins starting at lane*R.
 struct LaneBins {
     int kb[2];
@@ -253,348 +234,428 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
                                                  const int* __restrict__ krange, double* __restrict__ part)
 {
     __shared__ double sh[kMaxWaves][kPartStride];
+    // Z stencil exchange of the block: rows j0-2 .. j0+wpb+1 (J, H_r/J, H_Z/J) and the
+    // Z interfaces j0 .. j0+wpb (F_J, F_Hr, F_HZ), bin k of lane l at [q*64 + l].
+    __shared__ float zrow[kMaxWaves + 4][3][kMaxBins];
+    __shared__ float zflx[kMaxWaves + 1][3][kMaxBins];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const Seg sg = segs[(size_t)lb * wpb + wave];
     const int cidx = segs[(size_t)lb * wpb].cloud;
+    const int j0 = segs[(size_t)lb * wpb].j;
 
     double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
     KSum mf{0, 0}, Ls{0, 0}, net{0, 0}, outw{0, 0}, esc{0, 0};
 
-    if (sg.cloud >= 0 && sg.i0 < sg.i1) {
-        const Cloud& cl = clouds[sg.cloud];
-        const int nn = cl.nn, nr = cl.nr, nz = cl.nz, R = cl.R, j = sg.j;
-        const int hll = cl.hll, bcr = cl.bc_r_right, bcb = cl.bc_z_bot, bct = cl.bc_z_top;
-        const float* J = A.J[parity] + cl.o3;
-        const float* Hr = A.Hr[parity] + cl.o3;
-        const float* Hz = A.Hz[parity] + cl.o3;
-        float* Jw = A.J[parity ^ 1] + cl.o3;
-        float* Hrw = A.Hr[parity ^ 1] + cl.o3;
-        float* Hzw = A.Hz[parity ^ 1] + cl.o3;
-        const float* aF = A.aF + cl.o3;
-        const float* Hvp = A.Hv + cl.o3;
-        const float* php = A.phi + cl.o3;
-        const float* ivr = A.ivr + cl.oivr;
-        const float* ivz = A.ivz + cl.oivz;
-        const float* ucr = A.uc_r + cl.ofr;
-        const float* ucz = A.uc_z + cl.ofz;
-        const CellRec* rec = A.rec + cl.oc;
-        const BinRec* bins = A.bin + cl.obin;
-        const int kmin = krange[2 * sg.cloud], kmax = krange[2 * sg.cloud + 1];
+    const Cloud& cl = clouds[cidx];
+    const int nn = cl.nn, nr = cl.nr, nz = cl.nz, R = cl.R, j = sg.j;
+    const bool row_valid = (j < nz);                    // rows above the cloud are ghost rows
+    const int nvalid = min(wpb, nz - j0);
+    const int hll = cl.hll, bcr = cl.bc_r_right, bcb = cl.bc_z_bot, bct = cl.bc_z_top;
+    const float* J = A.J[parity] + cl.o3;
+    const float* Hr = A.Hr[parity] + cl.o3;
+    const float* Hz = A.Hz[parity] + cl.o3;
+    float* Jw = A.J[parity ^ 1] + cl.o3;
+    float* Hrw = A.Hr[parity ^ 1] + cl.o3;
+    float* Hzw = A.Hz[parity ^ 1] + cl.o3;
+    const float* aF = A.aF + cl.o3;
+    const float* Hvp = A.Hv + cl.o3;
+    const float* php = A.phi + cl.o3;
+    const float* ivr = A.ivr + cl.oivr;
+    const float* ivz = A.ivz + cl.oivz;
+    const float* ucr = A.uc_r + cl.ofr;
+    const float* ucz = A.uc_z + cl.ofz;
+    const CellRec* rec = A.rec + cl.oc;
+    const BinRec* bins = A.bin + cl.obin;
+    const int kmin = krange[2 * cidx], kmax = krange[2 * cidx + 1];
 
-        LaneBins L;
+    LaneBins L;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        L.kb[q] = lane * R + q;
+        L.val[q] = (q < R) && (L.kb[q] < nn);
+        L.act[q] = L.val[q] && L.kb[q] >= kmin && L.kb[q] <= kmax;
+    }
+
+    const double ct = cl.c_tilde, dt = cl.dt, cdt = ct * dt, dw = 1.0 / cdt;
+    const double inv_J0 = 1.0 / cl.J0;
+    const double fpc_ict = (kFourPi / kC) / ct * cl.J0;
+    const double inv_dt = 1.0 / dt;
+    const float cdtf = cl.cdt, cdtcf = cl.cdtc;
+    const int i0 = sg.i0;
+
+    // Ghost-aware (J, H_r/J, H_Z/J) of cell (i, row) for the Z exchange; rows beyond
+    // N_Z + 1 are never part of a needed stencil and are clamped.
+    auto load_zrow = [&](int i, int row, int slot) {
+        row = min(row, nz + 1);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            L.kb[q] = lane * R + q;
-            L.val[q] = (q < R) && (L.kb[q] < nn);
-            L.act[q] = L.val[q] && L.kb[q] >= kmin && L.kb[q] <= kmax;
+            if (!L.act[q]) continue;
+            float vj, vr, vz, fr, fz;
+            ld_z(J, Hr, Hz, nn, nr, nz, i, L.kb[q], row, bcb, bct, vj, vr, vz);
+            m1f::flux_ratios(vj, vr, vz, fr, fz);
+            const int p = q * 64 + lane;
+            zrow[slot][0][p] = vj;
+            zrow[slot][1][p] = fr;
+            zrow[slot][2][p] = fz;
         }
+    };
 
-        const double ct = cl.c_tilde, dt = cl.dt, cdt = ct * dt, dw = 1.0 / cdt;
-        const double inv_J0 = 1.0 / cl.J0;
-        const double fpc_ict = (kFourPi / kC) / ct * cl.J0;
-        const double inv_dt = 1.0 / dt;
-        const float cdtf = cl.cdt, cdtcf = cl.cdtc;
-        const int i0 = sg.i0;
+    // r window (J, H_r/J, H_Z/J) of cells (i-1, i, i+1, i+2); FL / kimR: flux and
+    // K_ϕϕ of the left interface of cell i.
+    float wJ[2][4], wR[2][4], wZ[2][4], FL[2][3], kimR[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            wJ[q][m] = 0.f; wR[q][m] = 0.f; wZ[q][m] = 0.f;
+            // cells i0-2, i0-1, i0 in slots 1..3: the first shift and load of the
+            // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1)
+            if (row_valid && L.val[q] && m >= 1) {
+                float vr, vz;
+                ld_r(J, Hr, Hz, nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m], vr, vz);
+                m1f::flux_ratios(wJ[q][m], vr, vz, wR[q][m], wZ[q][m]);
+            }
+        }
+        FL[q][0] = FL[q][1] = FL[q][2] = 0.f;
+        kimR[q] = 0.f;
+    }
 
-        // r window: cells (i-1, i, i+1, i+2) of the current step of the march;
-        // FL / kimR: flux and K_ϕϕ of the left interface of cell i.
-        float wJ[2][4], wR[2][4], wZ[2][4], FL[2][3], kimR[2];
+    // All wavefronts of the block march over the same cells [i0, i1) in lockstep;
+    // i = i0-1 only evaluates the left r interface of cell i0.
+    for (int i = i0 - 1; i < sg.i1; ++i) {
+        const bool first = (i < i0);
+        const size_t cell = (size_t)j * nr + i;   // used only for valid rows, i >= i0
+        const size_t base = cell * nn;
+
+        // ---- right r interface of cell i (FP32) ----
+        float FR[2][3], kip[2], kimN[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                wJ[q][m] = 0.f; wR[q][m] = 0.f; wZ[q][m] = 0.f;
-                // cells i0-2, i0-1, i0 in slots 1..3: the first shift and load of the
-                // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1)
-                if (L.val[q] && m >= 1)
-                    ld_r(J, Hr, Hz, nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m], wR[q][m], wZ[q][m]);
-            }
-            FL[q][0] = FL[q][1] = FL[q][2] = 0.f;
-            kimR[q] = 0.f;
-        }
-
-        // i = i0-1 only evaluates the left interface of cell i0.
-        for (int i = i0 - 1; i < sg.i1; ++i) {
-            const bool first = (i < i0);
-            const size_t cell = (size_t)j * nr + i;   // unused when first
-            const size_t base = cell * nn;
-            CellRec cr;
-            if (!first) cr = rec[cell];
-
-            double jo[2], hr[2], hz[2];
-            // ================= flux sweep (FP32) =================
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
-                if (!L.val[q]) continue;
-                const int k = L.kb[q];
-                wJ[q][0] = wJ[q][1]; wJ[q][1] = wJ[q][2]; wJ[q][2] = wJ[q][3];
-                wR[q][0] = wR[q][1]; wR[q][1] = wR[q][2]; wR[q][2] = wR[q][3];
-                wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
-                ld_r(J, Hr, Hz, nn, nr, j, k, i + 2, bcr, wJ[q][3], wR[q][3], wZ[q][3]);
-                const float Js = wJ[q][1], Hrs = wR[q][1], Hzs = wZ[q][1];
-                float Jn = Js, Hrn = Hrs, Hzn = Hzs;
-                if (L.act[q]) {
-                    float dJ = 0.f, dHr = 0.f, dHz = 0.f;   // Z part of the flux divergence
-                    float FR0 = 0.f, FR1 = 0.f, FR2 = 0.f, kip = 0.f, kimN = 0.f;
-                    const int nside = first ? 1 : 3;
-#pragma unroll 1
-                    for (int side = 0; side < nside; ++side) {
-                        m1f::Stencil32 st;
-                        const float* iv;
-                        float aL, aR, u;
-                        int dir;
-                        if (side == 0) {
-                            // right r interface of cell i: cells (i, i+1, i+2, i-1)
-                            dir = 0;
-                            st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
-                            st.Hr[0] = wR[q][1]; st.Hr[1] = wR[q][2]; st.Hr[2] = wR[q][3]; st.Hr[3] = wR[q][0];
-                            st.Hz[0] = wZ[q][1]; st.Hz[1] = wZ[q][2]; st.Hz[2] = wZ[q][3]; st.Hz[3] = wZ[q][0];
-                            iv = ivr + (size_t)(i + 1) * m1f::kIvStride;
-                            aL = aF[((size_t)j * nr + max(i, 0)) * nn + k];
-                            aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
-                            u = ucr[(size_t)j * (nr + 1) + i + 1];
-                        } else {
-                            // Z interface g (bottom g = j, top g = j+1): cells (g-1, g, g+1, g-2)
-                            dir = 1;
-                            const int g = j + side - 1;
-                            const int cells[4] = {g - 1, g, g + 1, g - 2};
-#pragma unroll
-                            for (int m = 0; m < 4; ++m)
-                                ld_z(J, Hr, Hz, nn, nr, nz, i, k, cells[m], bcb, bct, st.J[m], st.Hr[m], st.Hz[m]);
-                            iv = ivz + (size_t)g * m1f::kIvStride;
-                            aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
-                            aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
-                            u = ucz[(size_t)g * nr + i];
-                        }
-                        const m1f::Face32 f = m1f::interface_flux(dir, hll, st, iv, aL, aR, u);
-                        if (side == 0) {
-                            FR0 = f.FJ; FR1 = f.FHr; FR2 = f.FHz;
-                            kip = f.kphi_L;
-                            kimN = (i + 1 == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
-                        } else {
-                            const float gz = (side == 1) ? -cr.gZl : cr.gZr;
-                            dJ += gz * f.FJ;
-                            dHr += gz * f.FHr;
-                            dHz += gz * f.FHz;
-                        }
-                    }
-                    if (!first) {
-                        // cell update: src/AdvectJandH.jl:1319-1379
-                        const float gRl = cr.gRl, gRr = cr.gRr;
-                        Jn = Js - cdtf * ((gRr * FR0 - gRl * FL[q][0]) + dJ);
-                        Jn = m1f::fmaxf_(Jn, m1f::kJeps32);
-                        Hzn = Hzs - cdtf * ((gRr * FR2 - gRl * FL[q][2]) + dHz);
-                        Hrn = Hrs - cdtf * ((gRr * FR1 - gRl * FL[q][1]) + dHr);
-                        Hrn += cdtf * (0.5f * (kimR[q] + kip)) * cr.inv_r;
-                        if (cl.has_vel) {
-                            const float durdr = (float)cr.durdr, durdz = (float)cr.durdz;
-                            const float duzdr = (float)cr.duzdr, duzdz = (float)cr.duzdz, urr = (float)cr.urr;
-                            const m1f::Edd32 D = m1f::D_edd(Hrs, Hzs, Js);
-                            const float Dpp = m1f::fmaxf_(1.0f - D.rr - D.zz, 0.0f);
-                            const float Ddu = D.rr * durdr + D.rz * (durdz + duzdr) + D.zz * duzdz + Dpp * urr;
-                            Hrn += -cdtcf * (Hrs * durdr + Hzs * durdz);
-                            Hzn += -cdtcf * (Hrs * duzdr + Hzs * duzdz);
-                            Jn = m1f::fmaxf_(Jn - cdtcf * Ddu * Js, m1f::kJeps32);
-                        }
-                        // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
-                        m1f::causal_limit(Jn, Hrn, Hzn);
-                    }
-                    FL[q][0] = FR0; FL[q][1] = FR1; FL[q][2] = FR2;
-                    kimR[q] = kimN;
-                }
-                jo[q] = Jn; hr[q] = Hrn; hz[q] = Hzn;
-            }
-            if (first) continue;
-
-            // ================= emission / absorption (FP64) =================
-            // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462), J in units of J0
-            const double al_pd = cr.alpha * cr.pd, src = cr.js * ct * dt * inv_J0;
-            double hv[2] = {1.0, 1.0};
-            double sr = 0.0, sz = 0.0;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (!L.val[q]) continue;
-                const size_t idx = base + L.kb[q];
-                const double h = Hvp[idx];
-                hv[q] = h;
-                const double aJ = cr.alpha_c + al_pd * h;
-                const double jn = (jo[q] + src * (double)php[idx]) * drcp(1.0 + aJ * cdt);
-                const double iH = drcp(1.0 + (double)aF[idx] * cdt);
-                const double hrn = hr[q] * iH, hzn = hz[q] * iH;
-                const double dk = bins[L.kb[q]].dnu;
-                sr += dk * (hr[q] - hrn);
-                sz += dk * (hz[q] - hzn);
-                jo[q] = jn; hr[q] = hrn; hz[q] = hzn;
-            }
-            sr = wave_sum64(sr);
-            sz = wave_sum64(sz);
+            FR[q][0] = FR[q][1] = FR[q][2] = 0.f;
+            kip[q] = 0.f; kimN[q] = 0.f;
+            if (!row_valid || !L.val[q]) continue;
+            const int k = L.kb[q];
+            wJ[q][0] = wJ[q][1]; wJ[q][1] = wJ[q][2]; wJ[q][2] = wJ[q][3];
+            wR[q][0] = wR[q][1]; wR[q][1] = wR[q][2]; wR[q][2] = wR[q][3];
+            wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
             {
-                // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
-                const double fr = fpc_ict * sr * inv_dt, fz = fpc_ict * sz * inv_dt;
-                if (lane == 0) {
-                    A.f_r[cl.oc + cell] = fr;
-                    A.f_z[cl.oc + cell] = fz;
-                }
-                mf.add(cr.dV * sqrt(fr * fr + fz * fz));
-                Ls.add(kFourPi * cr.dV * cr.js);
-                const double ddr = cr.rcom, ddz = cr.zcom - cl.z_center;
-                const double dist = sqrt(ddr * ddr + ddz * ddz);
-                double frad = 0.0;
-                if (dist > 0.0) frad = (fr * ddr + fz * ddz) / dist;
-                net.add(cr.dV * frad);
-                outw.add(cr.dV * (frad > 0.0 ? frad : 0.0));
+                float vr, vz;
+                ld_r(J, Hr, Hz, nn, nr, j, k, i + 2, bcr, wJ[q][3], vr, vz);
+                m1f::flux_ratios(wJ[q][3], vr, vz, wR[q][3], wZ[q][3]);
             }
-
-            // ================= frequency factors (FP64) =================
-            // Per bin m: K_m, C_m and the fitting values of P_m (right half-width, the
-            // "dk" of row m and "dkm" of row m+1) and P'_m (left half-width, the "dkp"
-            // of row m-1).  Row k takes bin k-1's (A, oL) and bin k+1's (B', oR').
-            const bool vel = cl.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
-                                            cr.duzdz != 0.0 || cr.urr != 0.0);
-            const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
-            const double R0 = cl.recoil ? kHoverKB / cr.T : 0.0;
-            double Aq[2], Bq[2], oLq[2], oRq[2], Bpq[2], oRpq[2];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                Aq[q] = 0.0; Bq[q] = 0.0; oLq[q] = 1.0; oRq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
-                if (!L.val[q]) continue;
-                const BinRec bk = bins[L.kb[q]];
-                double VEL = 0.0;
-                if (vel) {
-                    double Dr, Dz, Dx;
-                    D_edd64(hr[q], hz[q], jo[q], Dr, Dz, Dx);
-                    double Dpp = 1.0 - Dr - Dz;
-                    Dpp = Dpp > 0.0 ? Dpp : 0.0;
-                    VEL = (Dr * cr.durdr + Dx * (cr.durdz + cr.duzdr) + Dz * cr.duzdz + Dpp * cr.urr) * kInvC;
-                }
-                const double K = PREF * hv[q];
-                const double iK = drcp(K);
-                double C = VEL * bk.nuC;
-                if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
-                const double qR = bk.half_R * iK, qL = bk.half_L * iK;
-                const double P = fmin(fmax(C * qR, -kPmax), kPmax);
-                const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
-                double oL, xL, oR, xR, oRp, xRp;
-                fit_both(P, oL, xL, oR, xR);
-                if (Pp == P) {
-                    oRp = oR; xRp = xR;
-                } else {
-                    fit_right(Pp, oRp, xRp);
-                }
-                Aq[q] = xL * qR;
-                Bq[q] = xR * qR;
-                oLq[q] = oL;
-                oRq[q] = oR;
-                Bpq[q] = xRp * qL;
-                oRpq[q] = oRp;
-            }
-            const int R1 = R - 1;
-            const double A_prev = __shfl_up(R1 ? Aq[1] : Aq[0], 1, 64);
-            const double oL_prev = __shfl_up(R1 ? oLq[1] : oLq[0], 1, 64);
-            const double Bp_next = __shfl_down(Bpq[0], 1, 64), oRp_next = __shfl_down(oRpq[0], 1, 64);
-            double ra[2], rb[2], rc[2], rd[2];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
-                const int k = L.kb[q];
-                if (!L.val[q] || k < 1 || k > nn - 2) continue;
-                const double Am = (q == 0) ? A_prev : Aq[0];
-                const double oLm = (q == 0) ? oL_prev : oLq[0];
-                const bool last = (q == R1);
-                const double Bpp = last ? Bp_next : Bpq[1];
-                const double oRpp = last ? oRp_next : oRpq[1];
-                const double idn = bins[k].inv_dnu;
-                const double den_kp = idn * drcp(Aq[q] - Bpp);
-                const double den_km = idn * drcp(Am - Bq[q]);
-                const double Fkp = oRpp * den_kp;
-                const double Fkm = oLm * den_km;
-                const double Fk = oLq[q] * den_kp + oRq[q] * den_km;
-                // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215); gtsv drops
-                // the couplings to bins 0 and N_ν-1, which keep their value
-                ra[q] = (k == 1) ? 0.0 : Fkm;
-                rc[q] = (k == nn - 2) ? 0.0 : Fkp;
-                rb[q] = -(Fk + dw);
-                rd[q] = -dw * jo[q];
-            }
-
-            // ================= tridiagonal solve: CR level + PCR over the lanes =================
-            double a, b, c, d, ua = 0.0, uc = 0.0, ud = 0.0;
-            if (R == 2) {
-                // eliminate the odd bin of each lane's pair into the even rows
-                const double ib1 = drcp(rb[1]);
-                ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
-                double pua = __shfl_up(ua, 1, 64), puc = __shfl_up(uc, 1, 64), pud = __shfl_up(ud, 1, 64);
-                if (lane == 0) { pua = 0.0; puc = 0.0; pud = 0.0; }
-                b = rb[0] - ra[0] * puc - rc[0] * ua;
-                d = rd[0] - ra[0] * pud - rc[0] * ud;
-                a = -ra[0] * pua;
-                c = -rc[0] * uc;
-            } else {
-                a = ra[0]; b = rb[0]; c = rc[0]; d = rd[0];
-            }
-#pragma unroll
-            for (int s = 1; s < 64; s <<= 1) {
-                const double ib = drcp(b);
-                const double u = a * ib, v = c * ib, w = d * ib;
-                double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
-                double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
-                if (lane < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
-                if (lane + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
-                const double nb = b - a * vb - c * uf;
-                const double nd = d - a * wb - c * wf;
-                a = -a * ub;
-                c = -c * vf;
-                b = nb;
-                d = nd;
-            }
-            double xs[2];
-            xs[0] = d * drcp(b);
-            xs[1] = 0.0;
-            if (R == 2) {
-                double xn = __shfl_down(xs[0], 1, 64);
-                if (lane == 63) xn = 0.0;
-                xs[1] = ud - ua * xs[0] - uc * xn;
-            }
-
-            // ================= H correction, store, reductions =================
-            const bool edge = (cr.A_rout != 0.0) || (cr.A_ztop != 0.0) || (cr.A_zbot != 0.0 && bcb == M1LYA_OUTFLOW);
-            const double Azb = (bcb == M1LYA_OUTFLOW) ? cr.A_zbot : 0.0;
-            double escl = 0.0;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (!L.val[q]) continue;
-                const int k = L.kb[q];
-                // J floor, H rescale and causality: src/UpdateLyaJandH.jl:335-379
-                const double jn = (k >= 1 && k <= nn - 2) ? xs[q] : jo[q];
-                const double Jk = jn > kJeps ? jn : kJeps;
-                const double ratio = Jk * drcp(jo[q] > kJeps ? jo[q] : kJeps);
-                double h_r = hr[q] * ratio, h_z = hz[q] * ratio;
-                const double Hm2 = h_r * h_r + h_z * h_z;
-                if (Hm2 > Jk * Jk) {
-                    const double s = Jk * drcp(sqrt(Hm2));
-                    h_r *= s;
-                    h_z *= s;
-                }
-                const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
-                const size_t idx = base + k;
-                Jw[idx] = Jf;
-                Hrw[idx] = hrf;
-                Hzw[idx] = hzf;
-                const double Jd = Jf, dk = bins[k].dnu;
-                e1[q] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
-                e10[q] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
-                // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
-                if (edge)
-                    escl += dk * kFourPi *
-                            (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
-                             Azb * fmax(-(double)hzf, 0.0));
-            }
-            if (edge) esc.add(wave_sum64(escl) * cl.J0);
+            if (!L.act[q]) continue;
+            m1f::Stencil32 st;
+            st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
+            st.fr[0] = wR[q][1]; st.fr[1] = wR[q][2]; st.fr[2] = wR[q][3]; st.fr[3] = wR[q][0];
+            st.fz[0] = wZ[q][1]; st.fz[1] = wZ[q][2]; st.fz[2] = wZ[q][3]; st.fz[3] = wZ[q][0];
+            const float aL = aF[((size_t)j * nr + max(i, 0)) * nn + k];
+            const float aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
+            const float u = ucr[(size_t)j * (nr + 1) + i + 1];
+            const m1f::Face32 f = m1f::interface_flux(0, hll, st, ivr + (size_t)(i + 1) * m1f::kIvStride, aL, aR, u);
+            FR[q][0] = f.FJ; FR[q][1] = f.FHr; FR[q][2] = f.FHz;
+            kip[q] = f.kphi_L;
+            kimN[q] = (i + 1 == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
         }
+        if (first) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
+                kimR[q] = kimN[q];
+            }
+            continue;
+        }
+
+        // ---- Z interfaces through LDS ----
+        if (row_valid) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (!L.act[q]) continue;
+                const int p = q * 64 + lane;
+                zrow[wave + 2][0][p] = wJ[q][1];
+                zrow[wave + 2][1][p] = wR[q][1];
+                zrow[wave + 2][2][p] = wZ[q][1];
+            }
+        } else {
+            load_zrow(i, j, wave + 2);
+        }
+        for (int h = wave; h < 4; h += wpb) {
+            const int slot = (h < 2) ? h : wpb + h;
+            load_zrow(i, j0 - 2 + slot, slot);
+        }
+        __syncthreads();
+        for (int f = wave; f <= nvalid; f += wpb) {
+            const int g = j0 + f;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (!L.act[q]) continue;
+                const int k = L.kb[q];
+                const int p = q * 64 + lane;
+                m1f::Stencil32 st;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int slot = f + ((m + 1) & 3);   // cells (g-1, g, g+1, g-2)
+                    st.J[m] = zrow[slot][0][p];
+                    st.fr[m] = zrow[slot][1][p];
+                    st.fz[m] = zrow[slot][2][p];
+                }
+                const float aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
+                const float aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+                const m1f::Face32 r = m1f::interface_flux(1, hll, st, ivz + (size_t)g * m1f::kIvStride, aL, aR,
+                                                          ucz[(size_t)g * nr + i]);
+                zflx[f][0][p] = r.FJ;
+                zflx[f][1][p] = r.FHr;
+                zflx[f][2][p] = r.FHz;
+            }
+        }
+        __syncthreads();
+        if (!row_valid) continue;
+
+        const CellRec cr = rec[cell];
+        double jo[2], hr[2], hz[2];
+        // ---- cell update (FP32): src/AdvectJandH.jl:1319-1379 ----
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
+            if (!L.val[q]) continue;
+            const int k = L.kb[q];
+            const float Js = wJ[q][1], Hrs = Hr[base + k], Hzs = Hz[base + k];
+            float Jn = Js, Hrn = Hrs, Hzn = Hzs;
+            if (L.act[q]) {
+                const int p = q * 64 + lane;
+                const float gRl = cr.gRl, gRr = cr.gRr, gZl = cr.gZl, gZr = cr.gZr;
+                const float dJ = gZr * zflx[wave + 1][0][p] - gZl * zflx[wave][0][p];
+                const float dHr = gZr * zflx[wave + 1][1][p] - gZl * zflx[wave][1][p];
+                const float dHz = gZr * zflx[wave + 1][2][p] - gZl * zflx[wave][2][p];
+                Jn = Js - cdtf * ((gRr * FR[q][0] - gRl * FL[q][0]) + dJ);
+                Jn = m1f::fmaxf_(Jn, m1f::kJeps32);
+                Hzn = Hzs - cdtf * ((gRr * FR[q][2] - gRl * FL[q][2]) + dHz);
+                Hrn = Hrs - cdtf * ((gRr * FR[q][1] - gRl * FL[q][1]) + dHr);
+                Hrn += cdtf * (0.5f * (kimR[q] + kip[q])) * cr.inv_r;
+                if (cl.has_vel) {
+                    const float durdr = (float)cr.durdr, durdz = (float)cr.durdz;
+                    const float duzdr = (float)cr.duzdr, duzdz = (float)cr.duzdz, urr = (float)cr.urr;
+                    const m1f::Edd32 D = m1f::D_edd(Hrs, Hzs, Js);
+                    const float Dpp = m1f::fmaxf_(1.0f - D.rr - D.zz, 0.0f);
+                    const float Ddu = D.rr * durdr + D.rz * (durdz + duzdr) + D.zz * duzdz + Dpp * urr;
+                    Hrn += -cdtcf * (Hrs * durdr + Hzs * durdz);
+                    Hzn += -cdtcf * (Hrs * duzdr + Hzs * duzdz);
+                    Jn = m1f::fmaxf_(Jn - cdtcf * Ddu * Js, m1f::kJeps32);
+                }
+                // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
+                m1f::causal_limit(Jn, Hrn, Hzn);
+            }
+            FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
+            kimR[q] = kimN[q];
+            jo[q] = Jn; hr[q] = Hrn; hz[q] = Hzn;
+        }
+
+        // ================= emission / absorption (FP64) =================
+        // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462), J in units of J0;
+        // the H deposit hr - hr/(1+x) is formed as hr·x/(1+x) (no cancellation)
+        const double al_pd = cr.alpha * cr.pd, src = cr.js * ct * dt * inv_J0;
+        double hv[2] = {1.0, 1.0};
+        double sr = 0.0, sz = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!L.val[q]) continue;
+            const size_t idx = base + L.kb[q];
+            const double h = Hvp[idx];
+            hv[q] = h;
+            const double aJ = cr.alpha_c + al_pd * h;
+            const double jn = (jo[q] + src * (double)php[idx]) * drcp1(1.0 + aJ * cdt);
+            const double xH = (double)aF[idx] * cdt;
+            const double gH = xH * drcp1(1.0 + xH);          // 1 - 1/(1+x)
+            const double dk = bins[L.kb[q]].dnu;
+            sr += dk * (hr[q] * gH);
+            sz += dk * (hz[q] * gH);
+            jo[q] = jn;
+            hr[q] -= hr[q] * gH;
+            hz[q] -= hz[q] * gH;
+        }
+        sr = wave_sum64(sr);
+        sz = wave_sum64(sz);
+        {
+            // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
+            const double fr = fpc_ict * sr * inv_dt, fz = fpc_ict * sz * inv_dt;
+            if (lane == 0) {
+                A.f_r[cl.oc + cell] = fr;
+                A.f_z[cl.oc + cell] = fz;
+            }
+            mf.add(cr.dV * sqrt(fr * fr + fz * fz));
+            Ls.add(kFourPi * cr.dV * cr.js);
+            const double ddr = cr.rcom, ddz = cr.zcom - cl.z_center;
+            const double d2 = ddr * ddr + ddz * ddz;
+            double frad = 0.0;
+            if (d2 > 0.0) frad = (fr * ddr + fz * ddz) * drcp1(sqrt(d2));
+            net.add(cr.dV * frad);
+            outw.add(cr.dV * (frad > 0.0 ? frad : 0.0));
+        }
+
+        // ================= frequency factors (FP64) =================
+        // Exponential fitting of src/FreqFactors.jl:142-278 in conservative interface
+        // form.  Per bin m: K_m, C_m, P_m over the right half-width (the "dk" of row
+        // m) and P'_m over the left half-width (the "dkp" of row m-1), oL_m = exp(-P_m),
+        // oR'_m = exp(P'_m), A_m = χL(P_m)·q_m, B'_m = χR(P'_m)·q'_m.  Per interface
+        // m+1/2: G = 1/(A_m - B'_{m+1}), shared by rows m and m+1, so the frequency
+        // flux leaving one bin is exactly the flux entering the next whatever the
+        // rounding of G.  Row k: Fkp = oR'_{k+1}·G_{k+1/2}/Δν_k, Fkm = oL_{k-1}·
+        // G_{k-1/2}/Δν_k, Fk = (oL_k·G_{k+1/2} + oR'_k·G_{k-1/2})/Δν_k.  (The reference
+        // row k+1 uses P_{k+1} where this uses P'_{k+1}; the two differ by the rounding
+        // of the bin centre, ~1e-12 relative.)
+        const bool vel = cl.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
+                                        cr.duzdz != 0.0 || cr.urr != 0.0);
+        const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
+        const double R0 = cl.recoil ? kHoverKB / cr.T : 0.0;
+        double Aq[2], oLq[2], Bpq[2], oRpq[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            Aq[q] = 1.0; oLq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
+            if (!L.val[q]) continue;
+            const BinRec bk = bins[L.kb[q]];
+            double VEL = 0.0;
+            if (vel) {
+                double Dr, Dz, Dx;
+                D_edd64(hr[q], hz[q], jo[q], Dr, Dz, Dx);
+                double Dpp = 1.0 - Dr - Dz;
+                Dpp = Dpp > 0.0 ? Dpp : 0.0;
+                VEL = (Dr * cr.durdr + Dx * (cr.durdz + cr.duzdr) + Dz * cr.duzdz + Dpp * cr.urr) * kInvC;
+            }
+            const double K = PREF * hv[q];
+            const double iK = drcp1(K);
+            double C = VEL * bk.nuC;
+            if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
+            const double qR = bk.half_R * iK, qL = bk.half_L * iK;
+            const double P = fmin(fmax(C * qR, -kPmax), kPmax);
+            const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
+            double oL, xL, oRp, xRp;
+            if (fabs(P) < kPmin) {
+                oL = 1.0 - P + 0.5 * (P * P);
+                xL = 1.0 - 0.5 * P;
+            } else {
+                oL = exp(-P);
+                xL = (1.0 - oL) * drcp0(P);
+            }
+            if (fabs(Pp) < kPmin) {
+                oRp = 1.0 + Pp + 0.5 * (Pp * Pp);
+                xRp = -1.0 - 0.5 * Pp;
+            } else {
+                // exp(P') = exp(P)·exp(P' - P); P' - P is the rounding of the bin centre
+                const double d = Pp - P;
+                oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp(Pp);
+                xRp = (1.0 - oRp) * drcp0(Pp);
+            }
+            Aq[q] = xL * qR;
+            oLq[q] = oL;
+            Bpq[q] = xRp * qL;
+            oRpq[q] = oRp;
+        }
+        const int R1 = R - 1;
+        // neighbours: bin k-1 -> oL (lane above for q = 0), bin k+1 -> B', oR' (lane below for q = R-1)
+        const double oL_prev = __shfl_up(R1 ? oLq[1] : oLq[0], 1, 64);
+        const double Bp_next = __shfl_down(Bpq[0], 1, 64), oRp_next = __shfl_down(oRpq[0], 1, 64);
+        // interfaces owned by this lane: Gq[q] = G at the top of bin lR+q
+        double Gq[2];
+        Gq[0] = drcp0(Aq[0] - ((R1 == 0) ? Bp_next : Bpq[1]));
+        Gq[1] = drcp0(Aq[1] - Bp_next);
+        const double G_prev = __shfl_up(R1 ? Gq[1] : Gq[0], 1, 64);
+        double ra[2], rb[2], rc[2], rd[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
+            const int k = L.kb[q];
+            if (!L.val[q] || k < 1 || k > nn - 2) continue;
+            const double oLm = (q == 0) ? oL_prev : oLq[0];
+            const double oRpp = (q == R1) ? oRp_next : oRpq[1];
+            const double Gm = (q == 0) ? G_prev : Gq[0];
+            const double Gp = Gq[q];
+            const double idn = bins[k].inv_dnu;
+            const double Fkp = oRpp * Gp * idn;
+            const double Fkm = oLm * Gm * idn;
+            const double Fk = (oLq[q] * Gp + oRpq[q] * Gm) * idn;
+            // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215); gtsv drops
+            // the couplings to bins 0 and N_ν-1, which keep their value
+            ra[q] = (k == 1) ? 0.0 : Fkm;
+            rc[q] = (k == nn - 2) ? 0.0 : Fkp;
+            rb[q] = -(Fk + dw);
+            rd[q] = -dw * jo[q];
+        }
+
+        // ================= tridiagonal solve: CR level + PCR over the lanes =================
+        double a, b, c, d, ua = 0.0, uc = 0.0, ud = 0.0;
+        if (R == 2) {
+            // eliminate the odd bin of each lane's pair into the even rows
+            const double ib1 = drcp1(rb[1]);
+            ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
+            double pua = __shfl_up(ua, 1, 64), puc = __shfl_up(uc, 1, 64), pud = __shfl_up(ud, 1, 64);
+            if (lane == 0) { pua = 0.0; puc = 0.0; pud = 0.0; }
+            b = rb[0] - ra[0] * puc - rc[0] * ua;
+            d = rd[0] - ra[0] * pud - rc[0] * ud;
+            a = -ra[0] * pua;
+            c = -rc[0] * uc;
+        } else {
+            a = ra[0]; b = rb[0]; c = rc[0]; d = rd[0];
+        }
+#pragma unroll
+        for (int s = 1; s < 64; s <<= 1) {
+            const double ib = drcp1(b);
+            const double u = a * ib, v = c * ib, w = d * ib;
+            double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
+            double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
+            if (lane < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
+            if (lane + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
+            const double nb = b - a * vb - c * uf;
+            const double nd = d - a * wb - c * wf;
+            a = -a * ub;
+            c = -c * vf;
+            b = nb;
+            d = nd;
+        }
+        double xs[2];
+        xs[0] = d * drcp1(b);
+        xs[1] = 0.0;
+        if (R == 2) {
+            double xn = __shfl_down(xs[0], 1, 64);
+            if (lane == 63) xn = 0.0;
+            xs[1] = ud - ua * xs[0] - uc * xn;
+        }
+
+        // ================= H correction, store, reductions =================
+        const bool edge = (cr.A_rout != 0.0) || (cr.A_ztop != 0.0) || (cr.A_zbot != 0.0 && bcb == M1LYA_OUTFLOW);
+        const double Azb = (bcb == M1LYA_OUTFLOW) ? cr.A_zbot : 0.0;
+        double escl = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!L.val[q]) continue;
+            const int k = L.kb[q];
+            // J floor, H rescale and causality: src/UpdateLyaJandH.jl:335-379
+            const double jn = (k >= 1 && k <= nn - 2) ? xs[q] : jo[q];
+            const double Jk = jn > kJeps ? jn : kJeps;
+            const double ratio = Jk * drcp0(jo[q] > kJeps ? jo[q] : kJeps);
+            double h_r = hr[q] * ratio, h_z = hz[q] * ratio;
+            const double Hm2 = h_r * h_r + h_z * h_z;
+            if (Hm2 > Jk * Jk) {
+                const double s = Jk * drcp1(sqrt(Hm2));
+                h_r *= s;
+                h_z *= s;
+            }
+            const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
+            const size_t idx = base + k;
+            Jw[idx] = Jf;
+            Hrw[idx] = hrf;
+            Hzw[idx] = hzf;
+            const double Jd = Jf, dk = bins[k].dnu;
+            e1[q] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
+            e10[q] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
+            // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
+            if (edge)
+                escl += dk * kFourPi *
+                        (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
+                         Azb * fmax(-(double)hzf, 0.0));
+        }
+        if (edge) esc.add(wave_sum64(escl) * cl.J0);
     }
 
     // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
@@ -639,7 +700,7 @@ __global__ __launch_bounds__(256) void k_active_rows(const Cloud* __restrict__ c
     const Cloud& cb = clouds[segs[(size_t)lb * wpb].cloud];
     const int nnb = cb.nn, R = cb.R;
     double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
-    if (sg.cloud >= 0) {
+    if (sg.j < cb.nz) {
         const float* J = A.J[parity] + cb.o3;
         for (int i = sg.i0; i < sg.i1; ++i) {
             const size_t cell = (size_t)sg.j * cb.nr + i;
@@ -902,13 +963,10 @@ void build_tiles(m1lya_batch* b)
         const int nseg_r = (cl.nr + b->seg_len - 1) / b->seg_len;
         for (int j0 = 0; j0 < cl.nz; j0 += b->wpb)
             for (int s = 0; s < nseg_r; ++s) {
-                for (int w = 0; w < b->wpb; ++w) {
-                    const int j = j0 + w;
-                    if (j < cl.nz)
-                        segs.push_back(Seg{c, j, s * b->seg_len, std::min(cl.nr, (s + 1) * b->seg_len)});
-                    else
-                        segs.push_back(Seg{-1, 0, 0, 0});
-                }
+                // rows j0+w >= N_Z are ghost rows: their wavefronts only take part in
+                // the Z exchange of the block
+                for (int w = 0; w < b->wpb; ++w)
+                    segs.push_back(Seg{c, j0 + w, s * b->seg_len, std::min(cl.nr, (s + 1) * b->seg_len)});
                 ++blk;
             }
         cl.blk1 = blk;

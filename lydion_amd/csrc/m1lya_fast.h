@@ -134,9 +134,18 @@ __device__ inline void hll_eigen(float f, float mu, float& l1, float& l3)
     l3 = fclamp((muf + FAC) * iSQ, -1.0f, 1.0f);
 }
 
+// Cells (i, i+1, i+2, i-1) of one interface: J and the flux ratios H/J.
 struct Stencil32 {
-    float J[4], Hr[4], Hz[4];   // cells (i, i+1, i+2, i-1) of the interface
+    float J[4], fr[4], fz[4];
 };
+
+// H/J of one cell (J floored as in the reference's reconstruction).
+__device__ __forceinline__ void flux_ratios(float J, float Hr, float Hz, float& fr, float& fz)
+{
+    const float iJ = rcp(fmaxf_(J, kJeps32));
+    fr = Hr * iJ;
+    fz = Hz * iJ;
+}
 
 struct Face32 {
     float FJ, FHr, FHz;
@@ -145,16 +154,15 @@ struct Face32 {
 };
 
 // ReconstructInterface -> closure -> GLF/HLL flux -> upwind advection flux
-// (src/AdvectJandH.jl:81-118, 155-204, 276-282, 359-473).  iv: kIvStride values.
+// (src/AdvectJandH.jl:81-118, 155-204, 276-282, 359-473).  iv: kIvStride values;
+// the stencil carries J and H/J of its four cells.
 __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, const float* iv, float aF_L,
                                         float aF_R, float u_over_c)
 {
     const float dL = iv[0], dR = iv[1], cL1 = iv[2], cL2 = iv[3], cR1 = iv[4], cR2 = iv[5];
     const float tau_L = aF_L * dL, tau_R = aF_R * dR;
-    const float iJ0 = rcp(fmaxf_(s.J[0], kJeps32)), iJ1 = rcp(fmaxf_(s.J[1], kJeps32));
-    const float iJ2 = rcp(fmaxf_(s.J[2], kJeps32)), iJ3 = rcp(fmaxf_(s.J[3], kJeps32));
-    const float fr0 = s.Hr[0] * iJ0, fr1 = s.Hr[1] * iJ1, fr2 = s.Hr[2] * iJ2, fr3 = s.Hr[3] * iJ3;
-    const float fz0 = s.Hz[0] * iJ0, fz1 = s.Hz[1] * iJ1, fz2 = s.Hz[2] * iJ2, fz3 = s.Hz[3] * iJ3;
+    const float fr0 = s.fr[0], fr1 = s.fr[1], fr2 = s.fr[2], fr3 = s.fr[3];
+    const float fz0 = s.fz[0], fz1 = s.fz[1], fz2 = s.fz[2], fz3 = s.fz[3];
     // left state: x_i + minmod(slope_{i-1/2}, slope_{i+1/2})·ΔL, slopes times ΔL
     const float dJc = s.J[1] - s.J[0], dfrc = fr1 - fr0, dfzc = fz1 - fz0;
     float J_L = fmaxf_(s.J[0] + minmod(cL1 * (s.J[0] - s.J[3]), cL2 * dJc), kJeps32);
