@@ -229,7 +229,7 @@ struct LaneBins {
 // Block = wpb wavefronts, one row segment each; all segments of a block belong
 // to one cloud, whose partial row the block writes.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+__global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ clouds, Arrays A,
                                                  const Seg* __restrict__ segs, int nblk, int wpb, int parity,
                                                  const int* __restrict__ krange, double* __restrict__ part)
 {
@@ -246,10 +246,21 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
     const int cidx = segs[(size_t)lb * wpb].cloud;
     const int j0 = segs[(size_t)lb * wpb].j;
 
-    double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
-    KSum mf{0, 0}, Ls{0, 0}, net{0, 0}, outw{0, 0}, esc{0, 0};
-
+    // per-wavefront compensated sums of M_F, L, radial net/out and f_esc, kept by lane 0
+    // in LDS (s, c pairs) rather than in registers of every lane
+    __shared__ double sacc[kMaxWaves][2 * kNScal];
+    enum { S_MF = 0, S_L, S_NET, S_OUT, S_ESC };
+    if (lane < 2 * kNScal) sacc[wave][lane] = 0.0;
+    auto kadd = [&](int m, double x) {
+        if (lane == 0) {
+            const double s0 = sacc[wave][2 * m], t = s0 + x;
+            sacc[wave][2 * m + 1] += (fabs(s0) >= fabs(x)) ? (s0 - t) + x : (x - t) + s0;
+            sacc[wave][2 * m] = t;
+        }
+    };
     const Cloud& cl = clouds[cidx];
+    // the wavefront's row of sh accumulates the active-bin energies e1[k], e10[k]
+    for (int t = lane; t < 2 * cl.nn; t += 64) sh[wave][t] = 0.0;
     const int nn = cl.nn, nr = cl.nr, nz = cl.nz, R = cl.R, j = sg.j;
     const bool row_valid = (j < nz);                    // rows above the cloud are ghost rows
     const int nvalid = min(wpb, nz - j0);
@@ -484,14 +495,14 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
                 A.f_r[cl.oc + cell] = fr;
                 A.f_z[cl.oc + cell] = fz;
             }
-            mf.add(cr.dV * sqrt(fr * fr + fz * fz));
-            Ls.add(kFourPi * cr.dV * cr.js);
+            kadd(S_MF, cr.dV * sqrt(fr * fr + fz * fz));
+            kadd(S_L, kFourPi * cr.dV * cr.js);
             const double ddr = cr.rcom, ddz = cr.zcom - cl.z_center;
             const double d2 = ddr * ddr + ddz * ddz;
             double frad = 0.0;
             if (d2 > 0.0) frad = (fr * ddr + fz * ddz) * drcp1(sqrt(d2));
-            net.add(cr.dV * frad);
-            outw.add(cr.dV * (frad > 0.0 ? frad : 0.0));
+            kadd(S_NET, cr.dV * frad);
+            kadd(S_OUT, cr.dV * (frad > 0.0 ? frad : 0.0));
         }
 
         // ================= frequency factors (FP64) =================
@@ -647,34 +658,23 @@ __global__ __launch_bounds__(256, 2) void k_step(const Cloud* __restrict__ cloud
             Hrw[idx] = hrf;
             Hzw[idx] = hzf;
             const double Jd = Jf, dk = bins[k].dnu;
-            e1[q] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
-            e10[q] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
+            sh[wave][k] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
+            sh[wave][nn + k] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
             // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
             if (edge)
                 escl += dk * kFourPi *
                         (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
                          Azb * fmax(-(double)hzf, 0.0));
         }
-        if (edge) esc.add(wave_sum64(escl) * cl.J0);
+        if (edge) kadd(S_ESC, wave_sum64(escl) * cl.J0);
     }
 
     // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
     const Cloud& cb = clouds[cidx];
-    const int nnb = cb.nn, Rb = cb.R;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int k = lane * Rb + q;
-        if (q < Rb && k < nnb) {
-            sh[wave][k] = e1[q] * cb.J0;
-            sh[wave][nnb + k] = e10[q] * cb.J0;
-        }
-    }
+    const int nnb = cb.nn;
+    for (int t = lane; t < 2 * nnb; t += 64) sh[wave][t] *= cb.J0;
     if (lane == 0) {
-        sh[wave][2 * nnb + 0] = mf.value();
-        sh[wave][2 * nnb + 1] = Ls.value();
-        sh[wave][2 * nnb + 2] = net.value();
-        sh[wave][2 * nnb + 3] = outw.value();
-        sh[wave][2 * nnb + 4] = esc.value();
+        for (int m = 0; m < kNScal; ++m) sh[wave][2 * nnb + m] = sacc[wave][2 * m] + sacc[wave][2 * m + 1];
     }
     __syncthreads();
     const int ncol = 2 * nnb + kNScal;
@@ -860,7 +860,7 @@ struct m1lya_batch {
     bool krange_valid = false;
 
     // tile shape and the work / partial-row tables it implies
-    int seg_len = 32, wpb = 4;
+    int seg_len = 22, wpb = 4;
     std::vector<Seg> segs;
     Seg* d_segs = nullptr;
     size_t segs_cap = 0;
@@ -1584,8 +1584,10 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         b->profiling = false;
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         set_ctrl(b, nullptr, 0);
-        static const int seg_opts[] = {4, 8, 16, 32, 64};
-        static const int wpb_opts[] = {1, 2, 4};
+        // segment lengths: powers of two and the lengths that give 3 or 6 wavefronts per
+        // SIMD on 256-cell rows; 2 or 4 wavefronts per block
+        static const int seg_opts[] = {6, 8, 11, 16, 22, 32, 43, 64};
+        static const int wpb_opts[] = {2, 4};
         int best_s = b->seg_len, best_w = b->wpb;
         double best = 1e300;
         hipEvent_t e0, e1;

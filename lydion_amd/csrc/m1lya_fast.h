@@ -12,8 +12,10 @@
 //  - flux magnitudes are formed from H/J ratios, so |H|² never underflows;
 //  - the thick/thin blend uses 1 - ϵ = τ²/(num + τ²) and (1 - ϵ)/τ = τ/(num + τ²)
 //    (no cancellation, no division by τ);
-//  - the closure's a = (1 - χ)/2 is evaluated in a cancellation-free form, and the
-//    cell-centred closure switches to FP64 for f > 0.9.
+//  - the Levermore closure uses χ = (3 + 4f²)/(5 + 2s) = (5 - 2s)/3, s = √(4 - 3f²)
+//    (exact), so D = aI + b n⊗n with a = (s - 1)/3, b = 2 - s: one square root and no
+//    division; a carries an absolute error of a few 1e-8, which is what enters the
+//    fluxes D·J; the cell-centred closure switches to FP64 for f > 0.9.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,32 +40,29 @@ __device__ __forceinline__ float fmaxf_(float a, float b) { return (b > a) ? b :
 __device__ __forceinline__ float fminf_(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return (x > hi) ? hi : ((x < lo) ? lo : x); }
 
-// MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign.
+// MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign
+// (sign bits compared, so no product can underflow; a zero argument gives 0).
 __device__ __forceinline__ float minmod(float A, float B)
 {
-    const bool same = (A > 0.0f && B > 0.0f) || (A < 0.0f && B < 0.0f);
-    return same ? ((fabsf(A) < fabsf(B)) ? A : B) : 0.0f;
+    const bool same = (__float_as_int(A) ^ __float_as_int(B)) >= 0;
+    const float m = (fabsf(A) < fabsf(B)) ? A : B;
+    return same ? m : 0.0f;
 }
 
-// a = (1-χ)/2 and b = (3χ-1)/2 of the Levermore closure (src/M1Closure.jl:28-33), with
-// 1-χ = 2u (3/(s+1) + 2)/(5+2s), u = (1-f)(1+f), s = √(4-3f²): no cancellation as f -> 1.
-__device__ __forceinline__ void levermore_ab(float f, float& a, float& b)
+// a = (1-χ)/2 and b = (3χ-1)/2 of the Levermore closure (src/M1Closure.jl:28-33) from
+// f²: χ = (5 - 2s)/3 with s = √(4 - 3f²).
+__device__ __forceinline__ void levermore_ab(float f2, float& a, float& b)
 {
-    float u = (1.0f - f) * (1.0f + f);
-    float s = fsqrt(1.0f + 3.0f * u);
-    float inv = rcp(5.0f + 2.0f * s);
-    float one_minus_chi = 2.0f * u * (3.0f * rcp(s + 1.0f) + 2.0f) * inv;
-    a = 0.5f * one_minus_chi;
-    b = 1.0f - 1.5f * one_minus_chi;
+    const float s = fsqrt(4.0f - 3.0f * f2);
+    a = (s - 1.0f) * kThird;
+    b = 2.0f - s;
 }
 
 __device__ __forceinline__ void levermore_ab64(double f, double& a, double& b)
 {
-    double u = (1.0 - f) * (1.0 + f);
-    double s = sqrt(1.0 + 3.0 * u);
-    double one_minus_chi = 2.0 * u * (3.0 / (s + 1.0) + 2.0) / (5.0 + 2.0 * s);
-    a = 0.5 * one_minus_chi;
-    b = 1.0 - 1.5 * one_minus_chi;
+    const double s = sqrt(4.0 - 3.0 * f * f);
+    a = (s - 1.0) * (1.0 / 3.0);
+    b = 2.0 - s;
 }
 
 struct Edd32 { float rr, zz, rz; };
@@ -94,7 +93,8 @@ __device__ inline Edd32 D_edd(float Hr, float Hz, float J)
         return D;
     }
     float a, b;
-    levermore_ab(fclamp(f, kFmin32, kFmax32), a, b);
+    const float fc = fclamp(f, kFmin32, kFmax32);
+    levermore_ab(fc * fc, a, b);
     const float inf_ = rcp(f);
     const float nr = xr * inf_, nz = xz * inf_;
     const float bnr = b * nr;
@@ -111,7 +111,7 @@ __device__ inline Edd32 D_edd_f_n(float f, float nr, float nz)
     if (f > kFmin32) {
         f = fminf_(f, kFmax32);
         float a, b;
-        levermore_ab(f, a, b);
+        levermore_ab(f * f, a, b);
         float bnr = b * nr;
         D.rr = a + bnr * nr;
         D.zz = a + b * nz * nz;
@@ -120,6 +120,34 @@ __device__ inline Edd32 D_edd_f_n(float f, float nr, float nz)
         D.rr = kThird; D.zz = kThird; D.rz = 0.0f;
     }
     return D;
+}
+
+// One reconstructed side of an interface: limits |f| to 1 (src/AdvectJandH.jl:430-452),
+// then the direction n = f/|f| and the closure D = aI + b n⊗n (D_ϕϕ = a).
+struct Side32 {
+    float fr, fz, f, nr, nz, a, b;
+};
+
+__device__ __forceinline__ Side32 side_state(float fr, float fz)
+{
+    Side32 s;
+    const float f2 = fr * fr + fz * fz;
+    if (f2 > kFmin32 * kFmin32) {
+        const float i = frsq(f2);
+        if (f2 > 1.0f) {
+            fr *= i; fz *= i;
+            s.f = 1.0f; s.nr = fr; s.nz = fz;
+        } else {
+            s.f = f2 * i; s.nr = fr * i; s.nz = fz * i;
+        }
+        const float fc = fminf_(s.f, kFmax32);
+        levermore_ab(fc * fc, s.a, s.b);
+    } else {
+        s.f = 0.0f; s.nr = 0.0f; s.nz = 0.0f;
+        s.a = kThird; s.b = 0.0f;
+    }
+    s.fr = fr; s.fz = fz;
+    return s;
 }
 
 __device__ inline void hll_eigen(float f, float mu, float& l1, float& l3)
@@ -171,16 +199,12 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
     float fr_R = fr1 - minmod(cR2 * (fr2 - fr1), cR1 * dfrc);
     float fz_L = fz0 + minmod(cL1 * (fz0 - fz3), cL2 * dfzc);
     float fz_R = fz1 - minmod(cR2 * (fz2 - fz1), cR1 * dfzc);
-    float f_L = fsqrt(fr_L * fr_L + fz_L * fz_L);
-    if (f_L > 1.0f) { float i = rcp(f_L); fr_L *= i; fz_L *= i; f_L = 1.0f; }
-    float f_R = fsqrt(fr_R * fr_R + fz_R * fz_R);
-    if (f_R > 1.0f) { float i = rcp(f_R); fr_R *= i; fz_R *= i; f_R = 1.0f; }
-    float nr_L = 0.f, nz_L = 0.f, nr_R = 0.f, nz_R = 0.f;
-    if (f_L > 0.0f) { float i = rcp(f_L); nr_L = fr_L * i; nz_L = fz_L * i; }
-    if (f_R > 0.0f) { float i = rcp(f_R); nr_R = fr_R * i; nz_R = fz_R * i; }
-    const float Hr_L = fr_L * J_L, Hr_R = fr_R * J_R, Hz_L = fz_L * J_L, Hz_R = fz_R * J_R;
-    const Edd32 DL = D_edd_f_n(f_L, nr_L, nz_L);
-    const Edd32 DR = D_edd_f_n(f_R, nr_R, nz_R);
+    const Side32 SL = side_state(fr_L, fz_L), SR = side_state(fr_R, fz_R);
+    const float Hr_L = SL.fr * J_L, Hr_R = SR.fr * J_R, Hz_L = SL.fz * J_L, Hz_R = SR.fz * J_R;
+    Edd32 DL, DR;
+    DL.rr = SL.a + SL.b * SL.nr * SL.nr; DL.zz = SL.a + SL.b * SL.nz * SL.nz; DL.rz = SL.b * SL.nr * SL.nz;
+    DR.rr = SR.a + SR.b * SR.nr * SR.nr; DR.zz = SR.a + SR.b * SR.nz * SR.nz; DR.rz = SR.b * SR.nr * SR.nz;
+    const float f_L = SL.f, f_R = SR.f, nr_L = SL.nr, nz_L = SL.nz, nr_R = SR.nr, nz_R = SR.nz;
     float FJ_L, FJ_R, FHr_L, FHr_R, FHz_L, FHz_R;
     if (dir == 0) {
         FJ_L = Hr_L; FJ_R = Hr_R;
@@ -223,8 +247,9 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
     r.FHr = FHr + u_over_c * (up ? Hr_R : Hr_L);
     r.FHz = FHz + u_over_c * (up ? Hz_R : Hz_L);
     r.J_L = J_L; r.J_R = J_R; r.Hz_R = Hz_R;
-    r.kphi_L = fclamp(1.0f - DL.rr - DL.zz, 0.0f, 1.0f) * J_L;
-    r.kphi_R = fclamp(1.0f - DR.rr - DR.zz, 0.0f, 1.0f) * J_R;
+    // clamp(1 - D_rr - D_ZZ, 0, 1)·J with 1 - D_rr - D_ZZ = D_ϕϕ = a
+    r.kphi_L = fclamp(SL.a, 0.0f, 1.0f) * J_L;
+    r.kphi_R = fclamp(SR.a, 0.0f, 1.0f) * J_R;
     return r;
 }
 
