@@ -115,10 +115,35 @@ struct KSum {   // Neumaier compensated sum
     __device__ double value() const { return s + c; }
 };
 
+// Cross-lane moves through DPP (register to register).  CTRL: 0x138 wave_shr:1 (lane l
+// receives lane l-1), 0x130 wave_shl:1 (lane l receives lane l+1), 0x110+n row_shr:n
+// (within rows of 16 lanes); lanes without a source receive 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double from_prev_lane(double v) { return dpp_f64<0x138>(v); }   // 0 in lane 0
+__device__ __forceinline__ double from_next_lane(double v) { return dpp_f64<0x130>(v); }   // 0 in lane 63
+
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum over the 64 lanes, in every lane: scans within rows of 16, then the four row totals
+// (a fixed order, so the result does not depend on anything but the inputs).
 __device__ __forceinline__ double wave_sum64(double v)
 {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<0x111>(v);
+    v += dpp_f64<0x112>(v);
+    v += dpp_f64<0x114>(v);
+    v += dpp_f64<0x118>(v);
+    return (lane_value(v, 15) + lane_value(v, 31)) + (lane_value(v, 47) + lane_value(v, 63));
 }
 
 // 1/x from the hardware estimate (relative error ~2^-24): drcp0 as is, for values that
@@ -565,13 +590,13 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         }
         const int R1 = R - 1;
         // neighbours: bin k-1 -> oL (lane above for q = 0), bin k+1 -> B', oR' (lane below for q = R-1)
-        const double oL_prev = __shfl_up(R1 ? oLq[1] : oLq[0], 1, 64);
-        const double Bp_next = __shfl_down(Bpq[0], 1, 64), oRp_next = __shfl_down(oRpq[0], 1, 64);
+        const double oL_prev = from_prev_lane(R1 ? oLq[1] : oLq[0]);
+        const double Bp_next = from_next_lane(Bpq[0]), oRp_next = from_next_lane(oRpq[0]);
         // interfaces owned by this lane: Gq[q] = G at the top of bin lR+q
         double Gq[2];
         Gq[0] = drcp0(Aq[0] - ((R1 == 0) ? Bp_next : Bpq[1]));
         Gq[1] = drcp0(Aq[1] - Bp_next);
-        const double G_prev = __shfl_up(R1 ? Gq[1] : Gq[0], 1, 64);
+        const double G_prev = from_prev_lane(R1 ? Gq[1] : Gq[0]);
         double ra[2], rb[2], rc[2], rd[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -600,8 +625,7 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
             // eliminate the odd bin of each lane's pair into the even rows
             const double ib1 = drcp1(rb[1]);
             ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
-            double pua = __shfl_up(ua, 1, 64), puc = __shfl_up(uc, 1, 64), pud = __shfl_up(ud, 1, 64);
-            if (lane == 0) { pua = 0.0; puc = 0.0; pud = 0.0; }
+            const double pua = from_prev_lane(ua), puc = from_prev_lane(uc), pud = from_prev_lane(ud);
             b = rb[0] - ra[0] * puc - rc[0] * ua;
             d = rd[0] - ra[0] * pud - rc[0] * ud;
             a = -ra[0] * pua;
@@ -609,8 +633,21 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         } else {
             a = ra[0]; b = rb[0]; c = rc[0]; d = rd[0];
         }
+        {
+            // level 1: neighbours through DPP
+            const double ib = drcp1(b);
+            const double u = a * ib, v = c * ib, w = d * ib;
+            const double ub = from_prev_lane(u), vb = from_prev_lane(v), wb = from_prev_lane(w);
+            const double uf = from_next_lane(u), vf = from_next_lane(v), wf = from_next_lane(w);
+            const double nb = b - a * vb - c * uf;
+            const double nd = d - a * wb - c * wf;
+            a = -a * ub;
+            c = -c * vf;
+            b = nb;
+            d = nd;
+        }
 #pragma unroll
-        for (int s = 1; s < 64; s <<= 1) {
+        for (int s = 2; s < 64; s <<= 1) {
             const double ib = drcp1(b);
             const double u = a * ib, v = c * ib, w = d * ib;
             double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
@@ -628,9 +665,7 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         xs[0] = d * drcp1(b);
         xs[1] = 0.0;
         if (R == 2) {
-            double xn = __shfl_down(xs[0], 1, 64);
-            if (lane == 63) xn = 0.0;
-            xs[1] = ud - ua * xs[0] - uc * xn;
+            xs[1] = ud - ua * xs[0] - uc * from_next_lane(xs[0]);
         }
 
         // ================= H correction, store, reductions =================
