@@ -18,7 +18,8 @@
 //     reduction across the 64 lanes;
 //   * H rescale and causality, FP32 store into the other state buffer;
 //   * Neumaier-compensated FP64 accumulation of M_F, L, f_esc and the active-bin
-//     energies: per block (k_step), per cloud (k_reduce), then k_final.
+//     energies: per block (k_step), per slice of a cloud's blocks (k_reduce), then per
+//     cloud (k_final).
 //
 // Work items (one per wavefront) and partial-sum rows (one per block) cover every
 // cloud of the batch, so uneven clouds share one launch.  The three launches of a
@@ -53,6 +54,7 @@ constexpr int kMaxBins = 128;
 constexpr int kNScal = 5;                        // MF, L, net, out, esc
 constexpr int kPartStride = 2 * kMaxBins + 8;    // e1[N_ν], e10[N_ν], scalars
 constexpr int kMaxWaves = 4;
+constexpr int kSlices = 16;                      // row slices of the first reduction stage
 
 // Per-cell inputs of the step, one 160-byte record per cell (read with scalar loads).
 struct CellRec {
@@ -86,7 +88,7 @@ struct Seg {
 
 struct Ctrl {
     double* hist;                  // [cap][n_clouds][8]
-    int cap, step;
+    int cap, step, next;           // history cursor of the running step / of the next step
 };
 
 struct Arrays {
@@ -769,20 +771,26 @@ __global__ __launch_bounds__(256) void k_active_rows(const Cloud* __restrict__ c
     }
 }
 
-// Per-cloud sums of the block rows: lanes over columns, wavefronts over rows,
-// Neumaier sums combined in a fixed order (deterministic).
+// Per-cloud sums of the block rows, stage 1: block (column chunk, cloud, slice) sums
+// one slice of the cloud's rows; lanes over columns, wavefronts over rows, Neumaier
+// sums combined in a fixed order (deterministic).  Block (0,0,0) also advances the
+// history cursor to the value the previous step's k_final left.
 __global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds, const double* __restrict__ part,
-                                                double* __restrict__ red)
+                                                double* __restrict__ red, Ctrl* __restrict__ ctrl)
 {
     __shared__ double ss[4][64], sc[4][64];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) ctrl->step = ctrl->next;
     const Cloud& cl = clouds[blockIdx.y];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + lane;
     const int ncol = 2 * cl.nn + kNScal;
     if ((int)blockIdx.x * 64 >= ncol) return;
+    const int nrow = cl.blk1 - cl.blk0;
+    const int per = (nrow + kSlices - 1) / kSlices;
+    const int r0 = cl.blk0 + (int)blockIdx.z * per, r1 = min(cl.blk1, r0 + per);
     KSum acc{0, 0};
     if (col < ncol)
-        for (int b = cl.blk0 + wave; b < cl.blk1; b += 4) acc.add(part[(size_t)b * kPartStride + col]);
+        for (int r = r0 + wave; r < r1; r += 4) acc.add(part[(size_t)r * kPartStride + col]);
     ss[wave][lane] = acc.s;
     sc[wave][lane] = acc.c;
     __syncthreads();
@@ -791,61 +799,70 @@ __global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds
         t.add(ss[1][lane]);
         t.add(ss[2][lane]);
         t.add(ss[3][lane]);
-        red[(size_t)blockIdx.y * kPartStride + col] = t.value();
+        red[((size_t)blockIdx.y * kSlices + blockIdx.z) * kPartStride + col] = t.value();
     }
 }
 
-// FindActiveBins (src/UpdateLyaJandH.jl:47-97) from the reduced energies.
-__device__ void active_range(const double* r, int nn, int kcmin, int kcmax, int* kr)
-{
-    double Etot = 0.0;
-    for (int k = 0; k < nn; ++k) Etot += r[k];
-    int kmn = nn - 1, kmx = 0;
-    for (int k = 0; k < nn; ++k)
-        if (r[nn + k] > kEpsActive * Etot) {
-            kmn = min(kmn, k);
-            kmx = max(kmx, k);
-        }
-    kr[0] = min(kmn, kcmin);
-    kr[1] = max(kmx, kcmax);
-}
-
-// Step scalars -- Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673)
-// and Compute_f_esc_Lyα_Cylindrical (:969-993) -- then the next step's active bins.
-// One block; thread 0 advances the history cursor after every cloud is written.
+// Step scalars, one block per cloud: the slices are summed per column, then
+// Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673),
+// Compute_f_esc_Lyα_Cylindrical (:969-993) and FindActiveBins
+// (src/UpdateLyaJandH.jl:47-97) for the next step.
 __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds, int n, const double* __restrict__ red,
                                                int* __restrict__ krange, double* __restrict__ scalars,
                                                Ctrl* __restrict__ ctrl, int active_only)
 {
-    const int step = ctrl->step;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const Cloud& cl = clouds[c];
-        const double* r = red + (size_t)c * kPartStride;
-        const int nn = cl.nn;
-        if (!active_only) {
-            double MF = r[2 * nn], L = r[2 * nn + 1], nt = r[2 * nn + 2], ow = r[2 * nn + 3];
-            const double es = r[2 * nn + 4];
-            if (L > 0.0) {
-                const double s = 1.0 / (L / kC);
-                MF *= s; nt *= s; ow *= s;
-            } else {
-                MF = 0.0; nt = 0.0; ow = 0.0;
-            }
-            const double Ltot = (cl.bc_z_bot == M1LYA_REFLECT) ? 2.0 * L : L;
-            double fesc = 0.0;
-            if (Ltot > 0.0) {
-                const double Lsim = (cl.bc_z_bot == M1LYA_REFLECT) ? 0.5 * Ltot : Ltot;
-                fesc = (Lsim > 0.0) ? es / Lsim : 0.0;
-            }
-            const double v[8] = {MF, nt, ow, Ltot, fesc, (double)krange[2 * c], (double)krange[2 * c + 1], 0.0};
-            for (int m = 0; m < 8; ++m) scalars[(size_t)c * 8 + m] = v[m];
-            if (ctrl->hist && step < ctrl->cap)
-                for (int m = 0; m < 8; ++m) ctrl->hist[((size_t)step * n + c) * 8 + m] = v[m];
-        }
-        active_range(r, nn, cl.kcore_min, cl.kcore_max, krange + 2 * c);
+    __shared__ double col[kPartStride];
+    __shared__ double tree[256];
+    __shared__ int kmn_s, kmx_s;
+    const int c = blockIdx.x;
+    const Cloud& cl = clouds[c];
+    const int nn = cl.nn, ncol = 2 * nn + kNScal, t = threadIdx.x;
+    for (int m = t; m < ncol; m += blockDim.x) {
+        KSum acc{0, 0};
+        for (int z = 0; z < kSlices; ++z) acc.add(red[((size_t)c * kSlices + z) * kPartStride + m]);
+        col[m] = acc.value();
+    }
+    if (t == 0) { kmn_s = nn - 1; kmx_s = 0; }
+    __syncthreads();
+    // Etot = Σ e1[k]: fixed-shape tree
+    tree[t] = (t < nn) ? col[t] : 0.0;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (t < w) tree[t] += tree[t + w];
+        __syncthreads();
+    }
+    const double Etot = tree[0];
+    if (t < nn && col[nn + t] > kEpsActive * Etot) {
+        atomicMin(&kmn_s, t);
+        atomicMax(&kmx_s, t);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && !active_only) ctrl->step = step + 1;
+    if (t != 0) return;
+    const int used0 = krange[2 * c], used1 = krange[2 * c + 1];
+    if (!active_only) {
+        double MF = col[2 * nn], L = col[2 * nn + 1], nt = col[2 * nn + 2], ow = col[2 * nn + 3];
+        const double es = col[2 * nn + 4];
+        if (L > 0.0) {
+            const double s = 1.0 / (L / kC);
+            MF *= s; nt *= s; ow *= s;
+        } else {
+            MF = 0.0; nt = 0.0; ow = 0.0;
+        }
+        const double Ltot = (cl.bc_z_bot == M1LYA_REFLECT) ? 2.0 * L : L;
+        double fesc = 0.0;
+        if (Ltot > 0.0) {
+            const double Lsim = (cl.bc_z_bot == M1LYA_REFLECT) ? 0.5 * Ltot : Ltot;
+            fesc = (Lsim > 0.0) ? es / Lsim : 0.0;
+        }
+        const double v[8] = {MF, nt, ow, Ltot, fesc, (double)used0, (double)used1, 0.0};
+        for (int m = 0; m < 8; ++m) scalars[(size_t)c * 8 + m] = v[m];
+        const int step = ctrl->step;
+        if (ctrl->hist && step < ctrl->cap)
+            for (int m = 0; m < 8; ++m) ctrl->hist[((size_t)step * n + c) * 8 + m] = v[m];
+        if (c == 0) ctrl->next = step + 1;
+    }
+    krange[2 * c] = min(kmn_s, cl.kcore_min);
+    krange[2 * c + 1] = max(kmx_s, cl.kcore_max);
 }
 
 __global__ void k_scale(float* a, float* b, float* c, size_t n, float s)
@@ -906,7 +923,7 @@ struct m1lya_batch {
     int* krange = nullptr;
     double* scalars = nullptr;
     Ctrl* ctrl = nullptr;
-    Ctrl h_ctrl{nullptr, 0, 0};
+    Ctrl h_ctrl{nullptr, 0, 0, 0};
     double* hist = nullptr;
     int hist_cap = 0;
 
@@ -1041,7 +1058,7 @@ void push_records(m1lya_batch* b, int cloud)
 
 void set_ctrl(m1lya_batch* b, double* hist, int cap)
 {
-    b->h_ctrl = Ctrl{hist, cap, 0};
+    b->h_ctrl = Ctrl{hist, cap, 0, 0};
     bchk(b, hipMemcpy(b->ctrl, &b->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice), "ctrl");
 }
 
@@ -1108,13 +1125,13 @@ void launch_step_kernels(m1lya_batch* b, int parity)
     }
     {
         BProf p(b, "k_reduce");
-        hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n), dim3(256), 0, s, b->d_cl, b->part,
-                           b->red);
+        hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n, kSlices), dim3(256), 0, s, b->d_cl,
+                           b->part, b->red, b->ctrl);
         p.done();
     }
     {
         BProf p(b, "k_final");
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 0);
+        hipLaunchKernelGGL(k_final, dim3(b->n), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 0);
         p.done();
     }
 }
@@ -1126,10 +1143,10 @@ void compute_krange(m1lya_batch* b)
     hipLaunchKernelGGL(k_active_rows, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs, b->nblk,
                        b->wpb, b->parity, b->part);
     bchk(b, hipGetLastError(), "k_active_rows");
-    hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n), dim3(256), 0, s, b->d_cl, b->part,
-                       b->red);
+    hipLaunchKernelGGL(k_reduce, dim3((2 * kMaxBins + kNScal + 63) / 64, b->n, kSlices), dim3(256), 0, s, b->d_cl,
+                       b->part, b->red, b->ctrl);
     bchk(b, hipGetLastError(), "k_reduce");
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 1);
+    hipLaunchKernelGGL(k_final, dim3(b->n), dim3(256), 0, s, b->d_cl, b->n, b->red, b->krange, b->scalars, b->ctrl, 1);
     bchk(b, hipGetLastError(), "k_final");
     b->krange_valid = true;
 }
@@ -1260,7 +1277,7 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
         A.f_r = balloc<double>(b, b->nc);
         A.f_z = balloc<double>(b, b->nc);
         b->d_cl = balloc<Cloud>(b, n_clouds);
-        b->red = balloc<double>(b, (size_t)n_clouds * kPartStride);
+        b->red = balloc<double>(b, (size_t)n_clouds * kSlices * kPartStride);
         b->krange = balloc<int>(b, (size_t)n_clouds * 2);
         b->scalars = balloc<double>(b, (size_t)n_clouds * 8);
         b->ctrl = balloc<Ctrl>(b, 1);
