@@ -1,6 +1,6 @@
 """Benchmark of the multi-frequency M1 Lyα step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64] [--workload cone|sweep]
 
 Workload: the 256×256 r-Z cloud with 128 frequency bins (τ0 = 1e8, T = 1e4 K,
 polar cone of half-angle π/4, τ_abs = 1, point source of radius 0.5 cells) in a
@@ -15,6 +15,11 @@ library's autotuner before the timed region.  Beside it the FP64 plain step
 (bit-identical to the reference restatement) is timed on the same state, and
 the accuracy block states the FP32 step's error in M_F, f_esc and the
 frequency-integrated J against the FP64 step after the same number of steps.
+
+--workload sweep: a batch of 64 seeded static clouds of uneven grid size
+(32..256 per side), optical depth (τ0 1e5..1e9), temperature, dust and
+frequency resolution (32..128 bins), stepped by one batched launch per step,
+beside the same clouds run one after another through the FP64 plain step.
 
 Metric: cell-frequency updates per second = N_r·N_Z·N_ν·K / wall time, state
 resident in HBM.  The JSON line carries the roofline of the dominant kernel
@@ -55,6 +60,9 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    p.add_argument("--workload", choices=("cone", "sweep"), default="cone",
+                   help="cone: the 256x256x128 cone cloud; sweep: the seeded batch of 64 uneven clouds")
+    p.add_argument("--clouds", type=int, default=64, help="sweep: number of clouds")
     p.add_argument("--n-r", type=int, default=256)
     p.add_argument("--n-z", type=int, default=256)
     p.add_argument("--n-nu", type=int, default=128)
@@ -130,12 +138,12 @@ def run_fp64(cloud, steps, warmup):
     return wall, stats
 
 
-def run_fp32(cloud, steps, warmup, autotune):
-    """Timed batched FP32 steps (a batch of this cloud).  The timed region launches the
+def run_fp32(clouds, steps, warmup, autotune):
+    """Timed batched FP32 steps of a batch of clouds.  The timed region launches the
     kernels eagerly with HIP events around each one (no host synchronisation); a second,
     untimed-for-the-headline pass replays the captured hipGraph."""
     from lydion_amd.batch import M1LyaBatch
-    b = M1LyaBatch([cloud])
+    b = M1LyaBatch(clouds)
     tiles = b.autotune(trial_steps=3) if autotune else (*b.tiles(), None)
     b.run_steps(warmup, history=False)
     b.reset_kernel_stats()
@@ -154,8 +162,9 @@ def run_fp32(cloud, steps, warmup, autotune):
     b.run_steps(steps, history=False)
     b.synchronize()
     graph_wall = time.perf_counter() - t1
-    if not np.all(np.isfinite(b.download(0, "J"))):
-        raise RuntimeError("non-finite J after the FP32 steps")
+    for idx in range(len(clouds)):
+        if not np.all(np.isfinite(b.download(idx, "J"))):
+            raise RuntimeError(f"non-finite J in cloud {idx} after the FP32 steps")
     b.close()
     return wall, stats, graph_wall, tiles
 
@@ -191,12 +200,105 @@ def roofline(stats, dom, upd_per_step, steps, workload):
             "kernels_ms_per_step": {k: round(v[1] / steps, 4) for k, v in sorted(stats.items())}}
 
 
+SWEEP_WORKLOAD = ("sweep of {n} seeded static clouds (cloud i from PCG64({seed}+i)): N_r, N_Z uniform in "
+                  "{{32,64,128,256}}, tau0 log-uniform in [1e5,1e9], T log-uniform in [1e2,1e4] K, tau_abs uniform "
+                  "in [0,10], N_nu uniform in {{32,64,128}}, point or uniform source; one batched launch per step")
+
+
+def cpu_baseline_sweep(clouds, seconds: float):
+    """The FP64 oracle stepping the sweep's clouds in order, two steps per cloud per pass,
+    passes repeated until the time bound."""
+    from oracle.oracle import OracleState
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    done, el, visits = 0, 0.0, 0
+    while el < seconds:
+        for cloud in clouds:
+            o = OracleState(cloud)
+            t0 = time.perf_counter()
+            o.step()
+            o.step()
+            el += time.perf_counter() - t0
+            done += 2 * cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
+            visits += 1
+            if el >= seconds:
+                break
+    return {"value": done / el, "unit": "cell-freq updates/s", "cores": threads, "kind": "port",
+            "sample": f"{visits} visits of 2 FP64 steps over the sweep's clouds in order ({el:.1f} s, "
+                      f"OMP_NUM_THREADS={threads})"}
+
+
+def main_sweep(args):
+    """The batched sweep: FP32 batch of all clouds vs the same clouds run one after another
+    through the FP64 plain step."""
+    from lydion_amd import m1lya as M
+    from lydion_amd import workloads as W
+    from lydion_amd.batch import M1LyaBatch
+    from lydion_amd.static_cloud import DEFAULT_SEED, developed_state
+
+    clouds = [developed_state(c) for c in W.sweep_clouds(args.clouds, DEFAULT_SEED)]
+    upd = W.updates_per_step(clouds)
+    wall, stats, graph_wall, tiles = run_fp32(clouds, args.steps, args.warmup, not args.no_autotune)
+    value = upd * args.steps / wall
+
+    # the same clouds, one FP64 plain run after another (each timed alone, summed)
+    steps64 = max(5, args.steps // 10)
+    wall64 = 0.0
+    for cloud in clouds:
+        w, _ = run_fp64(cloud, steps64, 2)
+        wall64 += w
+    seq64 = upd * steps64 / wall64
+
+    # accuracy of the batch against each cloud's FP64 step after the same steps
+    n_acc = args.accuracy_steps
+    b = M1LyaBatch(clouds)
+    h32 = b.run_steps(n_acc)
+    b.close()
+    err_mf, err_fesc = 0.0, 0.0
+    for idx, cloud in enumerate(clouds):
+        ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+        h64 = M.run_steps(ctx, n_acc, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot)
+        ctx.close()
+        rel = lambda a, r: float(np.max(np.abs(a - r) / np.maximum(np.abs(r), 1e-300)))
+        err_mf = max(err_mf, rel(h32[:, idx, 0], h64[:, 0]))
+        err_fesc = max(err_fesc, rel(h32[:, idx, 4], h64[:, 4]))
+
+    step_gbs = STEP_ALGO_BYTES["fp32"] * value / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline_sweep(clouds, args.cpu_seconds)
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-freq updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded developed radiation field per cloud, every bin active)",
+        "config": {"workload": SWEEP_WORKLOAD.format(n=len(clouds), seed=DEFAULT_SEED), "clouds": len(clouds),
+                   "updates_per_step": upd, "precision": "fp32", "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "step_algo_bytes_per_update": STEP_ALGO_BYTES["fp32"]},
+        "roofline": roofline(stats, "k_step", upd, args.steps, f"sweep{len(clouds)}_fp32"),
+        "cpu_baseline": cpu,
+        "fp64_sequential": {"value": seq64, "ms_per_step": 1e3 * wall64 / steps64, "steps": steps64,
+                            "note": "each cloud through its own FP64 context, one after another"},
+        "speedup_vs_fp64_sequential": value / seq64,
+        "graph_ms_per_step": 1e3 * graph_wall / args.steps,
+        "tiles": {"seg_len": tiles[0], "waves_per_block": tiles[1]},
+        "accuracy": {"steps": n_acc, "M_F_max_rel_err": err_mf, "f_esc_max_rel_err": err_fesc,
+                     "over": f"all {len(clouds)} clouds"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding of a cloud is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "sweep":
+        if args.precision != "fp32":
+            print("bench.py: the sweep workload measures the FP32 batch (with the FP64 sequential runs beside it)",
+                  file=sys.stderr)
+            sys.exit(2)
+        main_sweep(args)
+        return
 
     import lydion_amd as L
     from lydion_amd.static_cloud import developed_state
@@ -217,7 +319,7 @@ def main():
                                  upd_per_step, fp64_steps, f"cone_cloud_{grid}_fp64")}
 
     if args.precision == "fp32":
-        wall, stats, graph_wall, tiles = run_fp32(cloud, args.steps, args.warmup, not args.no_autotune)
+        wall, stats, graph_wall, tiles = run_fp32([cloud], args.steps, args.warmup, not args.no_autotune)
         value = upd_per_step * args.steps / wall
         dom = "k_step"
         workload = f"cone_cloud_{grid}_fp32"

@@ -18,7 +18,7 @@ module M1LyaGPU
 
 export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
-       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!
+       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!
 
 const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
 
@@ -67,6 +67,26 @@ download!(ctx, field, A::AbstractArray{Float64}) = GC.@preserve A check(
           ctx, field, A, length(A)), ctx)
 optr(A) = A === nothing ? Ptr{Float64}(C_NULL) : pointer(A)
 
+# The pointers in these structs are only valid while the caller GC-preserves g / Lyα.
+cgeometry(g) = CGeometry(pointer(g.r), pointer(g.Z), pointer(g.rCoM), pointer(g.ZCoM), pointer(g.ΔV),
+                         pointer(g.inv_ΔV), pointer(g.A_r), pointer(g.A_Z), pointer(g.inv_r_cell),
+                         pointer(g.Δr_L), pointer(g.Δr_R), pointer(g.inv_dr_i_im1), pointer(g.inv_dr_ip1_i),
+                         pointer(g.inv_dr_ip2_ip1),
+                         (g.Δr_L_InnerBoundary, g.Δr_R_InnerBoundary, g.inv_dr_i_im1_InnerBoundary,
+                          g.inv_dr_ip1_i_InnerBoundary, g.inv_dr_ip2_ip1_InnerBoundary),
+                         (g.Δr_L_OuterBoundary, g.Δr_R_OuterBoundary, g.inv_dr_i_im1_OuterBoundary,
+                          g.inv_dr_ip1_i_OuterBoundary, g.inv_dr_ip2_ip1_OuterBoundary),
+                         pointer(g.ΔZ_L), pointer(g.ΔZ_R), pointer(g.inv_dZ_j_jm1), pointer(g.inv_dZ_jp1_j),
+                         pointer(g.inv_dZ_jp2_jp1),
+                         (g.ΔZ_L_InnerBoundary, g.ΔZ_R_InnerBoundary, g.inv_dZ_j_jm1_InnerBoundary,
+                          g.inv_dZ_jp1_j_InnerBoundary, g.inv_dZ_jp2_jp1_InnerBoundary),
+                         (g.ΔZ_L_OuterBoundary, g.ΔZ_R_OuterBoundary, g.inv_dZ_j_jm1_OuterBoundary,
+                          g.inv_dZ_jp1_j_OuterBoundary, g.inv_dZ_jp2_jp1_OuterBoundary))
+
+cfreqgrid(Lyα) = CFreqGrid(pointer(Lyα.ν), pointer(Lyα.νC), pointer(Lyα.Δν), pointer(Lyα.inv_Δν),
+                           pointer(Lyα.inv_νC), Lyα.kcore_min - 1, Lyα.kcore_max - 1, Lyα.Recoil,
+                           Lyα.FluxMethod isa Val{:HLL} ? Cint(1) : Cint(0), Lyα.α_Lyα_floor)
+
 """Creates the device context of a LyαArrays + GeometryCylindrical and uploads
 the static coefficients and the state (zero-copy: Julia arrays are [k,i,j])."""
 function attach!(Lyα, g, T, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_Z_∂r, ∂u_Z_∂Z, u_r_over_r)
@@ -75,31 +95,12 @@ function attach!(Lyα, g, T, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_
     check(ccall((:m1lya_create, libm1lya), Cint, (Cint, Cint, Cint, Cint, Ref{Ptr{Cvoid}}),
                 N_ν, N_r, N_Z, 0, ref), C_NULL)
     ctx = ref[]
-    arrays = (g.r, g.Z, g.rCoM, g.ZCoM, g.ΔV, g.inv_ΔV, g.A_r, g.A_Z, g.inv_r_cell,
-              g.Δr_L, g.Δr_R, g.inv_dr_i_im1, g.inv_dr_ip1_i, g.inv_dr_ip2_ip1,
-              g.ΔZ_L, g.ΔZ_R, g.inv_dZ_j_jm1, g.inv_dZ_jp1_j, g.inv_dZ_jp2_jp1)
-    GC.@preserve arrays begin
-        cg = CGeometry(pointer(g.r), pointer(g.Z), pointer(g.rCoM), pointer(g.ZCoM), pointer(g.ΔV),
-                       pointer(g.inv_ΔV), pointer(g.A_r), pointer(g.A_Z), pointer(g.inv_r_cell),
-                       pointer(g.Δr_L), pointer(g.Δr_R), pointer(g.inv_dr_i_im1), pointer(g.inv_dr_ip1_i),
-                       pointer(g.inv_dr_ip2_ip1),
-                       (g.Δr_L_InnerBoundary, g.Δr_R_InnerBoundary, g.inv_dr_i_im1_InnerBoundary,
-                        g.inv_dr_ip1_i_InnerBoundary, g.inv_dr_ip2_ip1_InnerBoundary),
-                       (g.Δr_L_OuterBoundary, g.Δr_R_OuterBoundary, g.inv_dr_i_im1_OuterBoundary,
-                        g.inv_dr_ip1_i_OuterBoundary, g.inv_dr_ip2_ip1_OuterBoundary),
-                       pointer(g.ΔZ_L), pointer(g.ΔZ_R), pointer(g.inv_dZ_j_jm1), pointer(g.inv_dZ_jp1_j),
-                       pointer(g.inv_dZ_jp2_jp1),
-                       (g.ΔZ_L_InnerBoundary, g.ΔZ_R_InnerBoundary, g.inv_dZ_j_jm1_InnerBoundary,
-                        g.inv_dZ_jp1_j_InnerBoundary, g.inv_dZ_jp2_jp1_InnerBoundary),
-                       (g.ΔZ_L_OuterBoundary, g.ΔZ_R_OuterBoundary, g.inv_dZ_j_jm1_OuterBoundary,
-                        g.inv_dZ_jp1_j_OuterBoundary, g.inv_dZ_jp2_jp1_OuterBoundary))
+    GC.@preserve g begin
+        cg = cgeometry(g)
         check(ccall((:m1lya_set_geometry, libm1lya), Cint, (Ptr{Cvoid}, Ref{CGeometry}), ctx, cg), ctx)
     end
-    fm = Lyα.FluxMethod isa Val{:HLL} ? Cint(1) : Cint(0)
     GC.@preserve Lyα begin
-        fg = CFreqGrid(pointer(Lyα.ν), pointer(Lyα.νC), pointer(Lyα.Δν), pointer(Lyα.inv_Δν),
-                       pointer(Lyα.inv_νC), Lyα.kcore_min - 1, Lyα.kcore_max - 1, Lyα.Recoil, fm,
-                       Lyα.α_Lyα_floor)
+        fg = cfreqgrid(Lyα)
         check(ccall((:m1lya_set_freq_grid, libm1lya), Cint, (Ptr{Cvoid}, Ref{CFreqGrid}), ctx, fg), ctx)
     end
     for (f, A) in ((F_J, Lyα.J_Lyα), (F_J_OLD, Lyα.J_Lyα_old), (F_H_R, Lyα.H_r_Lyα), (F_H_Z, Lyα.H_Z_Lyα),
@@ -205,6 +206,64 @@ function Update_e_Lyα!(e_Lyα, Δν, J)
     check(ccall((:m1lya_update_e_lya, libm1lya), Cint, (Ptr{Cvoid},), ctx), ctx)
     download!(ctx, F_E_LYA, e_Lyα)
     return nothing
+end
+
+# --- mixed-precision batch (m1lya_batch_*): many static clouds per launch ---
+
+function bcheck(rc::Cint, b::Ptr{Cvoid})
+    rc == 0 && return nothing
+    buf = Vector{UInt8}(undef, 512)
+    ccall((:m1lya_batch_last_error, libm1lya), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Csize_t), b, buf, length(buf))
+    error("libm1lya batch status $rc: " * unsafe_string(pointer(buf)))
+end
+
+"""
+    BatchM1Lya(clouds) -> Ptr{Cvoid}
+
+`clouds` is a vector of named tuples `(Lyα, geometry, T, c̃, Δt, Z_center, BC_Z_bot)`,
+one per static cloud as `SetupStaticCloudTest` builds it (src/StaticCloudTest.jl:92-315).
+Returns the batch handle; step it with `run_batch!`.
+"""
+function BatchM1Lya(clouds)
+    n = length(clouds)
+    nn = Cint[size(c.Lyα.J_Lyα, 1) for c in clouds]
+    nr = Cint[size(c.Lyα.J_Lyα, 2) for c in clouds]
+    nz = Cint[size(c.Lyα.J_Lyα, 3) for c in clouds]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    bcheck(ccall((:m1lya_batch_create, libm1lya), Cint, (Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ref{Ptr{Cvoid}}),
+                 n, nn, nr, nz, h), C_NULL)
+    b = h[]
+    for (idx, c) in enumerate(clouds)
+        i = Cint(idx - 1); Lyα = c.Lyα
+        GC.@preserve Lyα c begin   # c.geometry, c.T and the Lyα arrays stay alive for the ccalls
+            cg = cgeometry(c.geometry)
+            bcheck(ccall((:m1lya_batch_set_geometry, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ref{CGeometry}), b, i, cg), b)
+            fg = cfreqgrid(Lyα)
+            bcheck(ccall((:m1lya_batch_set_freq_grid, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ref{CFreqGrid}), b, i, fg), b)
+            J0 = max(maximum(Lyα.J_Lyα), 1e-300)
+            bcheck(ccall((:m1lya_batch_set_scale, libm1lya), Cint, (Ptr{Cvoid}, Cint, Float64), b, i, J0), b)
+            for (f, A) in ((F_J, Lyα.J_Lyα), (F_H_R, Lyα.H_r_Lyα), (F_H_Z, Lyα.H_Z_Lyα), (F_ALPHA_F, Lyα.α_F_Lyα),
+                           (F_HVOIGT, Lyα.Hvoigt), (F_PHI_NU, Lyα.ϕ_ν), (F_ALPHA_LYA, Lyα.α_Lyα),
+                           (F_ALPHA_C, Lyα.αc_Lyα), (F_PD, Lyα.pd), (F_JS, Lyα.∫dν_js_Lyα), (F_DNU_D, Lyα.ΔνD),
+                           (F_T, c.T))
+                bcheck(ccall((:m1lya_batch_upload, libm1lya), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Csize_t),
+                             b, i, f, A, length(A)), b)
+            end
+        end
+        bcheck(ccall((:m1lya_batch_set_step, libm1lya), Cint,
+                     (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Cint, Cint, Cint),
+                     b, i, c.c̃, c.Δt, c.Z_center, bc(:Outflow), bc(c.BC_Z_bot), bc(:Outflow)), b)
+    end
+    return b
+end
+
+"""Runs `nsteps` steps of every cloud; returns the [8, nclouds, nsteps] step scalars
+(M_F, M_F_radial_net, M_F_radial_out, L_Lyα, f_esc, kmin, kmax, 0)."""
+function run_batch!(b::Ptr{Cvoid}, nsteps::Integer)
+    n = ccall((:m1lya_batch_count, libm1lya), Cint, (Ptr{Cvoid},), b)
+    hist = Array{Float64}(undef, 8, n, nsteps)
+    bcheck(ccall((:m1lya_batch_run_steps, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), b, nsteps, hist), b)
+    return hist
 end
 
 end # module

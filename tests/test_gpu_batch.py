@@ -190,3 +190,39 @@ def test_scale_choice(lut):
     ha, hb = a.run_steps(10), b.run_steps(10)
     assert np.max(np.abs(ha[:, 0, 0] - hb[:, 0, 0]) / np.abs(ha[:, 0, 0])) < 1e-5
     assert _energy_err(cloud, b.download(0, "J"), a.download(0, "J")) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def sweep(lut):
+    from lydion_amd import workloads as W
+    return W.sweep_clouds(64, lut=lut)
+
+
+def test_sweep_of_64_clouds_vs_fp64(sweep):
+    """The seeded 64-cloud sweep (sides 32..256, τ0 1e5..1e9, T 1e2..1e4 K, τ_abs 0..10,
+    32..128 bins, point or uniform source) in one batch, from a developed state:
+    every cloud's M_F and f_esc against its own FP64 run at each of 10 steps."""
+    clouds = [developed_state(c) for c in sweep]
+    b = M1LyaBatch(clouds)
+    b.autotune(trial_steps=1)
+    hb = b.run_steps(10)
+    for idx, cl in enumerate(clouds):
+        h64, ctx = _fp64_history(cl, 10)
+        _check_history(hb[:, idx, :], h64, f"sweep cloud {idx}")
+        assert _energy_err(cl, b.download(idx, "J"), ctx.download("J")) <= FP32_TOL, idx
+        ctx.close()
+    b.close()
+
+
+def test_sweep_first_steps_from_rest(sweep):
+    """The first 16 sweep clouds from their initial state (J = 0, sources on), 40 steps:
+    M_F from step 5 on, f_esc once the FP64 value exceeds 1e-12."""
+    from lydion_amd import workloads as W
+    clouds = W.sweep_clouds(16)
+    b = M1LyaBatch(clouds)
+    hb = b.run_steps(40)
+    for idx, cl in enumerate(clouds):
+        h64, ctx = _fp64_history(cl, 40)
+        _check_history(hb[:, idx, :], h64, f"sweep cloud {idx} from rest", start=5, fesc_min=1e-12)
+        ctx.close()
+    b.close()
