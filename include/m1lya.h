@@ -219,6 +219,34 @@ int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
 int m1lya_run_steps(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
                     int bc_r_right, int bc_z_bot, int bc_z_top, double *history);
 
+/* ---- reverse-mode gradient of the fused step ------------------------------
+ * The reference has no adjoint; this entry point is what a ChainRules `rrule`
+ * around m1lya_run_steps calls.  It runs n_steps fused steps from the
+ * context's current state (Δρu starting at zero) and returns the gradient of
+ *
+ *   Obj = Σ_s Σ_{q<3} mf_seed[3 s + q] · (M_F, M_F_radial_net, M_F_radial_out)_q of step s
+ *       + Σ_k spec_seed[k] · S_k of the final state,
+ *
+ * S_k = 4π Σ_{outer faces} A max(H_out, 0) / L_sim, the escaping spectrum whose
+ * Δν-weighted sum is f_esc (Compute_f_esc_Lyα_Cylindrical,
+ * src/ConstantsAndFunctions.jl:969-993).  mf_seed [n_steps][3] and spec_seed
+ * [N_ν] may be NULL (no such term).  The steps are those of m1lya_run_steps;
+ * on return the context holds the state after n_steps and history (may be
+ * NULL) the [n_steps][8] step scalars.  Outputs (each may be NULL):
+ *   grad_state [3][N_Z][N_r][N_ν]  dObj/d(J, H_r, H_Z) of the initial state;
+ *   grad_cell  [5][N_Z][N_r]       dObj/d(α_Lyα, α_c, ∫js, ΔνD, T), where the T entry
+ *                                  is the direct (recoil) dependence only;
+ *   grad_bin   [3][N_Z][N_r][N_ν]  dObj/d(α_F, Hvoigt, ϕ_ν).
+ * The active-bin range of each step is held at its forward value.  At kinks
+ * (limiters, floors, clamps) the derivative of the branch taken is returned.
+ * Memory: states are checkpointed every checkpoint_every steps (0: ceil(√n))
+ * and one segment is replayed at a time: about (n/S + 2 S + 12) state-sized
+ * FP64 arrays of N_ν N_r N_Z each (S = checkpoint interval). */
+int m1lya_adjoint_run(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
+                      int bc_r_right, int bc_z_bot, int bc_z_top, const double *mf_seed,
+                      const double *spec_seed, int checkpoint_every, double *objective, double *history,
+                      double *grad_state, double *grad_cell, double *grad_bin);
+
 /* ---- mixed-precision batched step ---------------------------------------
  * A batch advances many clouds of uneven size (N_r, N_Z >= 3, 3 <= N_ν <= 128),
  * optical depth and frequency resolution by the same fused step as m1lya_step

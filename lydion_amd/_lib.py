@@ -57,7 +57,8 @@ EXPORTED = [
     "m1lya_find_active_bins", "m1lya_advect_lya", "m1lya_emission_absorption_update",
     "m1lya_update_freq_factors", "m1lya_frequency_update_all_cells", "m1lya_update_f_mf_l",
     "m1lya_update_f_mf_l_gravity", "m1lya_compute_f_esc", "m1lya_reset_drhou", "m1lya_update_e_lya",
-    "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_set_profiling",
+    "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_adjoint_run",
+    "m1lya_set_profiling",
     "m1lya_kernel_stats_count", "m1lya_kernel_stats", "m1lya_reset_kernel_stats", "m1lya_selftest_math",
     "m1lya_batch_create", "m1lya_batch_destroy", "m1lya_batch_last_error", "m1lya_batch_count",
     "m1lya_batch_set_geometry", "m1lya_batch_set_freq_grid", "m1lya_batch_set_scale", "m1lya_batch_get_scale",
@@ -108,6 +109,7 @@ def load() -> C.CDLL:
         "m1lya_edge_spectra": (i, [vp, _dp, _dp, sz]),
         "m1lya_step": (i, [vp, d, d, d, i, i, i, _dp]),
         "m1lya_run_steps": (i, [vp, i, d, d, d, i, i, i, _dp]),
+        "m1lya_adjoint_run": (i, [vp, i, d, d, d, i, i, i, _dp, _dp, i, _dp, _dp, _dp, _dp, _dp]),
         "m1lya_set_profiling": (i, [vp, i]),
         "m1lya_kernel_stats_count": (i, [vp]),
         "m1lya_kernel_stats": (i, [vp, i, C.c_char_p, sz, C.POINTER(C.c_long), _dp]),

@@ -1,0 +1,611 @@
+// m1lya_adjoint.hip -- reverse sweep of the FP64 fused step (m1lya_adjoint_run).
+//
+// The step is X_{n+1} = C(E(A(X_n))): advection A (AdvectJandH_Cylindrical!,
+// src/AdvectJandH.jl:526-1390), emission/absorption E
+// (LyαEmissionAbsorptionUpdate!, src/ConstantsAndFunctions.jl:483-573), and the
+// frequency update C (FrequencyUpdateJLyα_AllCells!, src/UpdateLyaJandH.jl:231-391)
+// with its flux correction; the scalars are Update_f_Lyα_MF_L_Lyα!
+// (src/ConstantsAndFunctions.jl:588-674) and the escaping spectrum of
+// Compute_f_esc_Lyα_Cylindrical (:969-993).  Given the cotangent of X_{n+1},
+// three kernels produce the cotangent of X_n and accumulate those of the
+// coefficient fields:
+//
+//   k_adj_column    one thread per cell: E and C in reverse.  Recomputes E(X1)
+//                   and the frequency-coupling rows, solves the transposed
+//                   tridiagonal system, contracts the row cotangents with the
+//                   local derivatives of each row (dual-number passes over its
+//                   inputs), and reverses the flux correction and emission.
+//   k_adj_cell      one thread per (bin, cell): the cell update of A in
+//                   reverse (floors, causality limit, aberration, D:∇u and
+//                   K_ϕϕ terms); writes the cotangent of each cell's
+//                   flux-divergence sums and the cell's local cotangent.
+//   k_adj_gather    one thread per (bin, cell): adds, over the eight
+//                   interfaces whose stencil holds the cell, the derivative of
+//                   the interface outputs along the cell's J, H_r, H_Z and α_F
+//                   contracted with the interface cotangents (which come from
+//                   the flux-divergence cotangents of the two cells it
+//                   separates).  Gather form: no atomics, deterministic.
+//
+// Bins outside the step's active range pass through A unchanged.  The active
+// range itself is piecewise constant in the inputs and is held fixed.
+#include <hip/hip_runtime.h>
+
+#include "m1lya_adjoint.h"
+#include "m1lya_internal.h"
+
+namespace m1 {
+
+using m1a::Dl;
+
+namespace {
+
+__device__ __forceinline__ void seed(double&, double) {}
+__device__ __forceinline__ void seed(Dl& x, double d) { x.d = d; }
+
+// Ghost-aware stencil load along r (the values of load_r in m1lya_kernels.hip),
+// with the tangent of the slot along variable `var` (0 J, 1 H_r, 2 H_Z, else
+// none) of cell `tc` of the same row.
+template <class T>
+__device__ __forceinline__ void tload_r(const double* J, const double* Hr, const double* Hz, int nn, int nr, int j,
+                                        int k, int c, int bc_right, int tc, int var, T& sJ, T& sHr, T& sHz)
+{
+    int src;
+    double hr_fac = 1.0;
+    bool clip = false;
+    if (c < 0) { src = -c - 1; hr_fac = -1.0; }
+    else if (c < nr) { src = c; }
+    else if (bc_right == BC_REFLECT) { src = 2 * nr - 1 - c; hr_fac = -1.0; }
+    else { src = nr - 1; clip = true; }
+    const size_t idx = ((size_t)j * nr + src) * nn + k;
+    const double vJ = J[idx], vHr = Hr[idx], vHz = Hz[idx];
+    sJ = T(vJ);
+    sHr = T(clip ? dmax(vHr, 0.0) : hr_fac * vHr);
+    sHz = T(vHz);
+    if (src == tc) {
+        if (var == 0) seed(sJ, 1.0);
+        else if (var == 1) seed(sHr, clip ? ((0.0 > vHr) ? 0.0 : 1.0) : hr_fac);
+        else if (var == 2) seed(sHz, 1.0);
+    }
+}
+
+// Same along Z (load_z), tc = row index of the target cell in column i.
+template <class T>
+__device__ __forceinline__ void tload_z(const double* J, const double* Hr, const double* Hz, int nn, int nr, int nz,
+                                        int i, int k, int c, int bc_bot, int bc_top, int tc, int var, T& sJ,
+                                        T& sHr, T& sHz)
+{
+    int src;
+    double hz_fac = 1.0;
+    int clip = 0;   // 1: max(H_Z, 0) (top outflow), 2: min(H_Z, 0) (bottom outflow)
+    if (c < 0) {
+        if (bc_bot == BC_REFLECT) { src = -c - 1; hz_fac = -1.0; }
+        else { src = 0; clip = 2; }
+    } else if (c < nz) {
+        src = c;
+    } else if (bc_top == BC_REFLECT) {
+        src = 2 * nz - 1 - c; hz_fac = -1.0;
+    } else {
+        src = nz - 1; clip = 1;
+    }
+    const size_t idx = ((size_t)src * nr + i) * nn + k;
+    const double vJ = J[idx], vHr = Hr[idx], vHz = Hz[idx];
+    sJ = T(vJ);
+    sHr = T(vHr);
+    double hz = hz_fac * vHz, dhz = hz_fac;
+    if (clip == 1) { hz = dmax(vHz, 0.0); dhz = (0.0 > vHz) ? 0.0 : 1.0; }
+    if (clip == 2) { hz = dmin(vHz, 0.0); dhz = (0.0 < vHz) ? 0.0 : 1.0; }
+    sHz = T(hz);
+    if (src == tc) {
+        if (var == 0) seed(sJ, 1.0);
+        else if (var == 1) seed(sHr, 1.0);
+        else if (var == 2) seed(sHz, dhz);
+    }
+}
+
+// Interface f of row j (r, dir 0) or column i (Z, dir 1) at bin k, with the
+// tangent along `var` of target cell tc (var 3: α_F of tc).
+template <class T>
+__device__ m1a::FaceT<T> face_at(const AdjAdvArgs& a, int dir, int f, int i, int j, int k, int tc, int var)
+{
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    m1a::StencilT<T> st;
+    const int cells[4] = {f - 1, f, f + 1, f - 2};
+    const double* iv;
+    double u;
+    T aFL, aFR;
+    if (dir == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            tload_r(a.J, a.Hr, a.Hz, nn, nr, j, k, cells[m], a.bc_r_right, tc, var, st.J[m], st.Hr[m], st.Hz[m]);
+        iv = a.g.ivr + (size_t)f * 5;
+        const int cl = max(f - 1, 0), cr = min(f, nr - 1);
+        aFL = T(a.aF[((size_t)j * nr + cl) * nn + k]);
+        aFR = T(a.aF[((size_t)j * nr + cr) * nn + k]);
+        if (var == 3) {
+            if (cl == tc) seed(aFL, 1.0);
+            if (cr == tc) seed(aFR, 1.0);
+        }
+        u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            tload_z(a.J, a.Hr, a.Hz, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, tc, var, st.J[m],
+                    st.Hr[m], st.Hz[m]);
+        iv = a.g.ivz + (size_t)f * 5;
+        const int cl = max(f - 1, 0), cr = min(f, nz - 1);
+        aFL = T(a.aF[((size_t)cl * nr + i) * nn + k]);
+        aFR = T(a.aF[((size_t)cr * nr + i) * nn + k]);
+        if (var == 3) {
+            if (cl == tc) seed(aFL, 1.0);
+            if (cr == tc) seed(aFR, 1.0);
+        }
+        u = a.v.uz_star[(size_t)f * nr + i] * kInvC;
+    }
+    return m1a::tface<T>(dir, a.hll, dir == 0 && f == 0, st, iv, aFL, aFR, u);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Cell update of AdvectJandH_Cylindrical! (src/AdvectJandH.jl:1302-1382) in reverse.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_cell(AdjAdvArgs a)
+{
+    const int nn = a.nn, nr = a.g.nr;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    const int j = blockIdx.z;
+    if (k >= nn || i >= nr) return;
+    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    const double lJ3 = a.lJ1[idx], lHr3 = a.lHr1[idx], lHz3 = a.lHz1[idx];
+    if (k < a.krange[0] || k > a.krange[1]) {
+        a.pJ[idx] = 0.0; a.pHr[idx] = 0.0; a.pHz[idx] = 0.0;
+        a.oJ[idx] = lJ3; a.oHr[idx] = lHr3; a.oHz[idx] = lHz3;
+        return;
+    }
+    // forward values of the four interfaces (as k_advect)
+    double Fr[2][3], Fz[2][3], kphi_im = 0.0, kphi_ip = 0.0;
+#pragma unroll 1
+    for (int side = 0; side < 4; ++side) {
+        const int dir = side < 2 ? 0 : 1;
+        const int f = dir == 0 ? i + side : j + side - 2;
+        m1a::FaceT<double> r = face_at<double>(a, dir, f, i, j, k, -1, -1);
+        double* F = dir == 0 ? Fr[side] : Fz[side - 2];
+        F[0] = r.FJ; F[1] = r.FHr; F[2] = r.FHz;
+        if (side == 0) kphi_im = r.kphi_R;
+        if (side == 1) kphi_ip = r.kphi_L;
+    }
+    const int c = j * nr + i;
+    const double Ar_L = a.g.A_r[(size_t)j * (nr + 1) + i], Ar_R = a.g.A_r[(size_t)j * (nr + 1) + i + 1];
+    const double Az_L = a.g.A_z[(size_t)j * nr + i], Az_R = a.g.A_z[(size_t)(j + 1) * nr + i];
+    const double cdt = a.cdt;
+    const double cdt_dV = cdt * a.g.inv_dV[c];
+    const double cdt_over_c = cdt * kInvC;
+    const double durdr = a.v.dur_dr[c], durdz = a.v.dur_dz[c], duzdr = a.v.duz_dr[c];
+    const double duzdz = a.v.duz_dz[c], urr = a.v.ur_over_r[c];
+    const double Js = a.J[idx], Hrs = a.Hr[idx], Hzs = a.Hz[idx];
+    Edd D = D_edd(Hrs, Hzs, Js);
+    const double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
+    const double Jn0 = Js - cdt_dV * ((Ar_R * Fr[1][0] - Ar_L * Fr[0][0]) + (Az_R * Fz[1][0] - Az_L * Fz[0][0]));
+    const double Jn1 = dmax(Jn0, kJeps);
+    const double Hzn0 = Hzs - cdt_dV * ((Ar_R * Fr[1][2] - Ar_L * Fr[0][2]) + (Az_R * Fz[1][2] - Az_L * Fz[0][2]));
+    const double Hrn0 = Hrs - cdt_dV * ((Ar_R * Fr[1][1] - Ar_L * Fr[0][1]) + (Az_R * Fz[1][1] - Az_L * Fz[0][1]));
+    const double Kav = 0.5 * (kphi_im + kphi_ip);
+    const double Hrn1 = Hrn0 + cdt * Kav * a.g.inv_r_cell[i];
+    const double Hrn2 = Hrn1 + (-cdt_over_c * (Hrs * durdr + Hzs * durdz));
+    const double Hzn2 = Hzn0 + (-cdt_over_c * (Hrs * duzdr + Hzs * duzdz));
+    const double Jm = Jn1 - cdt_over_c * Ddu * Js;
+    const double Jn2 = dmax(Jm, kJeps);
+
+    // EnforceCausalReducedFlux in reverse
+    double lHrn2 = lHr3, lHzn2 = lHz3, lJn2 = lJ3;
+    const double Hm2 = Hrn2 * Hrn2 + Hzn2 * Hzn2;
+    if (Hm2 > Jn2 * Jn2) {
+        const double hm = sqrt(Hm2);
+        const double fac = Jn2 / hm;
+        const double dot = Hrn2 * lHr3 + Hzn2 * lHz3;
+        const double g = Jn2 * dot / (hm * hm * hm);
+        lHrn2 = fac * lHr3 - g * Hrn2;
+        lHzn2 = fac * lHz3 - g * Hzn2;
+        lJn2 += dot / hm;
+    }
+    const double lJm = (Jm >= kJeps) ? lJn2 : 0.0;
+    const double lDdu = -lJm * cdt_over_c * Js;
+    double lJs = -lJm * cdt_over_c * Ddu;
+    const double lJn0 = (Jn0 >= kJeps) ? lJm : 0.0;
+    // aberration ΔH (src/AdvectJandH.jl:300-312)
+    double lHrs = -cdt_over_c * (lHrn2 * durdr + lHzn2 * duzdr);
+    double lHzs = -cdt_over_c * (lHrn2 * durdz + lHzn2 * duzdz);
+    // identity parts of the updates
+    lJs += lJn0;
+    lHrs += lHrn2;
+    lHzs += lHzn2;
+    // D:∇u of the cell's own closure
+    if (lDdu != 0.0) {
+#pragma unroll 1
+        for (int var = 0; var < 3; ++var) {
+            Dl tJ(Js, var == 0 ? 1.0 : 0.0), tHr(Hrs, var == 1 ? 1.0 : 0.0), tHz(Hzs, var == 2 ? 1.0 : 0.0);
+            m1a::EddT<Dl> Dd = m1a::tD_edd(tHr, tHz, tJ);
+            const double d = m1a::tD_colon_grad_u(Dd, durdr, durdz, duzdr, duzdz, urr).d;
+            if (var == 0) lJs += lDdu * d;
+            else if (var == 1) lHrs += lDdu * d;
+            else lHzs += lDdu * d;
+        }
+    }
+    a.pJ[idx] = lJn0;
+    a.pHr[idx] = lHrn2;   // the K_ϕϕ term and the flux divergence enter H_r with the same cotangent
+    a.pHz[idx] = lHzn2;
+    a.oJ[idx] = lJs;
+    a.oHr[idx] = lHrs;
+    a.oHz[idx] = lHzs;
+}
+
+// ---------------------------------------------------------------------------
+// Interface contributions gathered per cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_gather(AdjAdvArgs a)
+{
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    const int j = blockIdx.z;
+    if (k >= nn || i >= nr) return;
+    if (k < a.krange[0] || k > a.krange[1]) return;
+    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    const double cdt = a.cdt;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int t = 0; t < 8; ++t) {
+        const int dir = t < 4 ? 0 : 1;
+        const int f = dir == 0 ? i - 1 + t : j - 1 + (t - 4);
+        const int nf = dir == 0 ? nr : nz;
+        if (f < 0 || f > nf) continue;
+        // cotangents of the interface outputs from the two cells it separates
+        double lF[3] = {0.0, 0.0, 0.0}, lkL = 0.0, lkR = 0.0;
+        if (f >= 1) {
+            const int cl = dir == 0 ? j * nr + (f - 1) : (f - 1) * nr + i;
+            const double A = dir == 0 ? a.g.A_r[(size_t)j * (nr + 1) + f] : a.g.A_z[(size_t)f * nr + i];
+            const double s = -cdt * a.g.inv_dV[cl] * A;
+            const size_t q = (size_t)cl * nn + k;
+            lF[0] += s * a.pJ[q]; lF[1] += s * a.pHr[q]; lF[2] += s * a.pHz[q];
+            if (dir == 0) lkL = 0.5 * cdt * a.g.inv_r_cell[f - 1] * a.pHr[q];
+        }
+        if (f <= nf - 1) {
+            const int cr = dir == 0 ? j * nr + f : f * nr + i;
+            const double A = dir == 0 ? a.g.A_r[(size_t)j * (nr + 1) + f] : a.g.A_z[(size_t)f * nr + i];
+            const double s = cdt * a.g.inv_dV[cr] * A;
+            const size_t q = (size_t)cr * nn + k;
+            lF[0] += s * a.pJ[q]; lF[1] += s * a.pHr[q]; lF[2] += s * a.pHz[q];
+            if (dir == 0) lkR = 0.5 * cdt * a.g.inv_r_cell[f] * a.pHr[q];
+        }
+        if (lF[0] == 0.0 && lF[1] == 0.0 && lF[2] == 0.0 && lkL == 0.0 && lkR == 0.0) continue;
+        const int tc = dir == 0 ? i : j;
+#pragma unroll 1
+        for (int var = 0; var < 4; ++var) {
+            m1a::FaceT<Dl> r = face_at<Dl>(a, dir, f, i, j, k, tc, var);
+            acc[var] += lF[0] * r.FJ.d + lF[1] * r.FHr.d + lF[2] * r.FHz.d + lkL * r.kphi_L.d + lkR * r.kphi_R.d;
+        }
+    }
+    a.oJ[idx] += acc[0];
+    a.oHr[idx] += acc[1];
+    a.oHz[idx] += acc[2];
+    a.gaF[idx] += acc[3];
+}
+
+// ---------------------------------------------------------------------------
+// Emission/absorption and the frequency update in reverse, one thread per cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
+{
+    const int nn = a.nn, nr = a.nr;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= nr * a.nz) return;
+    const int i = c % nr, j = c / nr;
+    const size_t base = (size_t)c * nn;
+    const double ct = a.c_tilde, dt = a.dt, dw = a.dw;
+    const double fpc_ict = (kFourPi / kC) * (1.0 / ct);
+    const double ac = a.alpha_c[c], al = a.alpha[c], pd = a.pd[c], jsv = a.js[c];
+    const double* Hv = a.Hv + base;
+    const double* phi = a.phi + base;
+    const double* aF = a.aF + base;
+    double* X2J = a.X2J + base;
+    double* X2r = a.X2Hr + base;
+    double* X2z = a.X2Hz + base;
+    double* lJ = a.lJ1 + base;    // holds the cotangent of E's output until the emission reverse
+    double* lHr = a.lHr1 + base;
+    double* lHz = a.lHz1 + base;
+
+    // ---- E forward (values) and the momentum deposit Δρu ----
+    double sr = 0.0, sz = 0.0;
+    for (int k = 0; k < nn; ++k) {
+        const double aJ = ac + al * pd * Hv[k];
+        const double dJdt = jsv * ct * phi[k];
+        X2J[k] = (a.J1[base + k] + dJdt * dt) / (1.0 + aJ * ct * dt);
+        const double denH = 1.0 + aF[k] * ct * dt;
+        const double h1r = a.Hr1[base + k], h1z = a.Hz1[base + k];
+        X2r[k] = h1r / denH;
+        X2z[k] = h1z / denH;
+        sr += a.dnu[k] * (h1r - X2r[k]);
+        sz += a.dnu[k] * (h1z - X2z[k]);
+    }
+    // ---- cotangent of Δρu from the seeds of M_F, M_F_radial_net, M_F_radial_out ----
+    double lDr = 0.0, lDz = 0.0;
+    if (a.mf_scale != 0.0) {
+        const double fr = fpc_ict * sr / dt, fz = fpc_ict * sz / dt;
+        const double s = a.mf_scale * a.dV[c];
+        double lfr = 0.0, lfz = 0.0;
+        const double fm = sqrt(fr * fr + fz * fz);
+        if (fm > 0.0) { lfr += a.w0 * s * fr / fm; lfz += a.w0 * s * fz / fm; }
+        const double dr = a.rcom[i], dz = a.zcom[j] - a.z_center;
+        const double dist = sqrt(dr * dr + dz * dz);
+        if (dist > 0.0) {
+            const double er = dr / dist, ez = dz / dist;
+            const double frad = fr * er + fz * ez;
+            const double w = a.w1 + ((frad > 0.0) ? a.w2 : 0.0);
+            lfr += w * s * er;
+            lfz += w * s * ez;
+        }
+        lDr = lfr / dt;
+        lDz = lfz / dt;
+    }
+
+    const int n = nn - 2;
+    double* rA = a.rowA + base;
+    double* rB = a.rowB + base;
+    double* rC = a.rowC + base;
+    double* x = a.xs + base;
+    double* mu = a.mu + base;
+    double* cp = a.cp + base;
+    const double dnuD = a.dnuD[c];
+    const double amax = dmax(al, a.alpha_floor);
+    const double PREFACT = 0.5 * amax * (dnuD * dnuD);
+    const double Tc = a.T[c];
+    const double RECOIL_0 = kHoverKB / Tc;
+    const double durdr = a.dur_dr[c], durdz = a.dur_dz[c], duzdr = a.duz_dr[c], duzdz = a.duz_dz[c];
+    const double urr = a.ur_over_r[c];
+
+    auto row_const = [&](int k) {
+        m1a::RowConst rc;
+        rc.nuCk = a.nuC[k]; rc.nuCkp = a.nuC[k + 1]; rc.nuCkm = a.nuC[k - 1];
+        rc.dkp = rc.nuCkp - a.nu[k + 1];
+        rc.dk = a.nu[k + 1] - rc.nuCk;
+        rc.dkm = a.nu[k] - rc.nuCkm;
+        rc.inv_dnu_k = a.inv_dnu[k];
+        rc.two_inv_k = 2.0 * a.inv_nuC[k];
+        rc.two_inv_kp = 2.0 * a.inv_nuC[k + 1];
+        rc.two_inv_km = 2.0 * a.inv_nuC[k - 1];
+        rc.recoil = a.recoil;
+        return rc;
+    };
+
+    // ---- C forward: rows of the coupling matrix and the solution of the solve ----
+    for (int k = 1; k <= nn - 2; ++k) {
+        Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
+        const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
+        m1a::RowT<double> row = m1a::tfreq_row<double>(row_const(k), Hv[k - 1], Hv[k], Hv[k + 1], PREFACT,
+                                                        RECOIL_0, VEL);
+        rA[k - 1] = row.Fkm;
+        rB[k - 1] = -(row.Fk + dw);
+        rC[k - 1] = row.Fkp;
+    }
+    if (n >= 1) {
+        // Thomas on the M-matrix (the forward pivoted gtsv solves the same system)
+        cp[0] = rC[0] / rB[0];
+        x[0] = (-dw * X2J[1]) / rB[0];
+        for (int r = 1; r < n; ++r) {
+            const double m = rB[r] - rA[r] * cp[r - 1];
+            cp[r] = rC[r] / m;
+            x[r] = ((-dw * X2J[r + 1]) - rA[r] * x[r - 1]) / m;
+        }
+        for (int r = n - 2; r >= 0; --r) x[r] = x[r] - cp[r] * x[r + 1];
+    }
+
+    // ---- flux correction in reverse (src/UpdateLyaJandH.jl:335-379) ----
+    for (int k = 0; k < nn; ++k) {
+        const double J4 = a.J4[base + k];
+        const double J2 = X2J[k], Hr2 = X2r[k], Hz2 = X2z[k];
+        const double inv_J2 = 1.0 / dmax(J2, kJeps);
+        const double ratio = J4 * inv_J2;
+        const double hr = Hr2 * ratio, hz = Hz2 * ratio;
+        double lJ4 = a.lJ4[base + k], lhr = a.lHr4[base + k], lhz = a.lHz4[base + k];
+        const double Hm2 = hr * hr + hz * hz;
+        if (Hm2 > J4 * J4) {
+            const double hm = sqrt(Hm2);
+            const double s = J4 / hm;
+            const double dot = hr * lhr + hz * lhz;
+            const double g = J4 * dot / (hm * hm * hm);
+            lJ4 += dot / hm;
+            const double nhr = s * lhr - g * hr, nhz = s * lhz - g * hz;
+            lhr = nhr;
+            lhz = nhz;
+        }
+        const double lr = Hr2 * lhr + Hz2 * lhz;
+        lJ4 += lr * inv_J2;
+        double lJ2 = (J2 >= kJeps) ? -lr * ratio * inv_J2 : 0.0;
+        const double lJ3 = (J4 > kJeps) ? lJ4 : 0.0;
+        if (k == 0 || k == nn - 1) lJ2 += lJ3;   // bins outside the solve keep E's value
+        else mu[k - 1] = lJ3;                      // right-hand side of the transposed solve
+        lJ[k] = lJ2;
+        lHr[k] = ratio * lhr;
+        lHz[k] = ratio * lhz;
+    }
+
+    // ---- transposed tridiagonal solve: M^T mu = lJ3 ----
+    double lPREF = 0.0, lREC = 0.0;
+    if (n >= 1) {
+        // M^T: sub[r] = C[r-1], diag[r] = B[r], sup[r] = A[r+1]
+        cp[0] = (n > 1 ? rA[1] : 0.0) / rB[0];
+        mu[0] = mu[0] / rB[0];
+        for (int r = 1; r < n; ++r) {
+            const double m = rB[r] - rC[r - 1] * cp[r - 1];
+            cp[r] = (r + 1 < n ? rA[r + 1] : 0.0) / m;
+            mu[r] = (mu[r] - rC[r - 1] * mu[r - 1]) / m;
+        }
+        for (int r = n - 2; r >= 0; --r) mu[r] = mu[r] - cp[r] * mu[r + 1];
+
+        for (int k = 1; k <= nn - 2; ++k) {
+            const int r = k - 1;
+            const double m_r = mu[r];
+            lJ[k] += -dw * m_r;   // b = -dw J2
+            // cotangents of the matrix entries: dM = -mu x^T on the band
+            const double lFkm = (r >= 1) ? -m_r * x[r - 1] : 0.0;
+            const double lFk = m_r * x[r];                       // B = -(Fk + dw)
+            const double lFkp = (r <= n - 2) ? -m_r * x[r + 1] : 0.0;
+            if (lFkm == 0.0 && lFk == 0.0 && lFkp == 0.0) continue;
+            // local derivatives of the row over its inputs
+            Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
+            const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
+            const m1a::RowConst rc = row_const(k);
+            double g[6];
+#pragma unroll 1
+            for (int d = 0; d < 6; ++d) {
+                if (d == 4 && !a.recoil) { g[d] = 0.0; continue; }
+                Dl hm(Hv[k - 1], d == 0 ? 1.0 : 0.0), h0(Hv[k], d == 1 ? 1.0 : 0.0), hp(Hv[k + 1], d == 2 ? 1.0 : 0.0);
+                Dl pf(PREFACT, d == 3 ? 1.0 : 0.0), rec(RECOIL_0, d == 4 ? 1.0 : 0.0), vel(VEL, d == 5 ? 1.0 : 0.0);
+                m1a::RowT<Dl> row = m1a::tfreq_row<Dl>(rc, hm, h0, hp, pf, rec, vel);
+                g[d] = lFkp * row.Fkp.d + lFk * row.Fk.d + lFkm * row.Fkm.d;
+            }
+            a.g_Hv[base + k - 1] += g[0];
+            a.g_Hv[base + k] += g[1];
+            a.g_Hv[base + k + 1] += g[2];
+            lPREF += g[3];
+            lREC += g[4];
+            const double lDdu = g[5] * kInvC;
+            if (lDdu != 0.0) {
+#pragma unroll 1
+                for (int var = 0; var < 3; ++var) {
+                    Dl tJ(X2J[k], var == 0 ? 1.0 : 0.0), tHr(X2r[k], var == 1 ? 1.0 : 0.0);
+                    Dl tHz(X2z[k], var == 2 ? 1.0 : 0.0);
+                    m1a::EddT<Dl> Dd = m1a::tD_edd(tHr, tHz, tJ);
+                    const double dd = m1a::tD_colon_grad_u(Dd, durdr, durdz, duzdr, duzdz, urr).d;
+                    if (var == 0) lJ[k] += lDdu * dd;
+                    else if (var == 1) lHr[k] += lDdu * dd;
+                    else lHz[k] += lDdu * dd;
+                }
+            }
+        }
+    }
+
+    // ---- E in reverse (src/ConstantsAndFunctions.jl:451-573) ----
+    double lal = 0.0, lac = 0.0, ljs = 0.0;
+    for (int k = 0; k < nn; ++k) {
+        const double aJ = ac + al * pd * Hv[k];
+        const double den = 1.0 + aJ * ct * dt;
+        const double lJ2 = lJ[k];
+        lJ[k] = lJ2 / den;
+        const double q = lJ2 * dt * ct / den;
+        a.g_phi[base + k] += q * jsv;
+        ljs += q * phi[k];
+        const double laJ = -q * X2J[k];
+        lac += laJ;
+        lal += laJ * pd * Hv[k];
+        a.g_Hv[base + k] += laJ * al * pd;
+        const double denH = 1.0 + aF[k] * ct * dt;
+        const double dr_k = lDr * fpc_ict * a.dnu[k], dz_k = lDz * fpc_ict * a.dnu[k];
+        const double lHr2 = lHr[k], lHz2 = lHz[k];
+        lHr[k] = lHr2 / denH + dr_k * (1.0 - 1.0 / denH);
+        lHz[k] = lHz2 / denH + dz_k * (1.0 - 1.0 / denH);
+        a.g_aF[base + k] += -((lHr2 - dr_k) * X2r[k] + (lHz2 - dz_k) * X2z[k]) * ct * dt / denH;
+    }
+    // PREFACT = ½ max(α, α_floor) ΔνD², RECOIL_0 = h/(k_B T)
+    a.g_alpha[c] += lal + ((a.alpha_floor > al) ? 0.0 : lPREF * 0.5 * (dnuD * dnuD));
+    a.g_alpha_c[c] += lac;
+    a.g_js[c] += ljs;
+    a.g_dnuD[c] += lPREF * amax * dnuD;
+    a.g_T[c] += -lREC * kHoverKB / (Tc * Tc);
+}
+
+// Escaping luminosity per bin over L_sim (the integrand of f_esc,
+// src/ConstantsAndFunctions.jl:914-993): one block per bin.
+__global__ __launch_bounds__(256) void k_adj_spectrum(const double* Hr, const double* Hz, const double* A_r,
+                                                      const double* A_z, int nn, int nr, int nz, int bc_z_bot,
+                                                      double inv_Lsim, double* S)
+{
+    __shared__ double sh[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nz + 2 * nr; t += blockDim.x) {
+        if (t < nz) {
+            s += A_r[(size_t)t * (nr + 1) + nr] * kFourPi * dmax(Hr[((size_t)t * nr + nr - 1) * nn + k], 0.0);
+        } else if (t < nz + nr) {
+            const int i = t - nz;
+            s += A_z[(size_t)nz * nr + i] * kFourPi * dmax(Hz[((size_t)(nz - 1) * nr + i) * nn + k], 0.0);
+        } else if (bc_z_bot == BC_OUTFLOW) {
+            const int i = t - nz - nr;
+            s += A_z[i] * kFourPi * dmax(-Hz[(size_t)i * nn + k], 0.0);
+        }
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[k] = sh[0] * inv_Lsim;
+}
+
+// Seeds Σ_k v_k S_k of the final state into the state cotangent.
+__global__ __launch_bounds__(256) void k_adj_spectrum_seed(const double* Hr, const double* Hz, const double* A_r,
+                                                           const double* A_z, int nn, int nr, int nz, int bc_z_bot,
+                                                           double inv_Lsim, const double* v, double* lHr,
+                                                           double* lHz)
+{
+    const int k = blockIdx.x;
+    for (int t = threadIdx.x; t < nz + 2 * nr; t += blockDim.x) {
+        const double w = v[k] * kFourPi * inv_Lsim;
+        if (t < nz) {
+            const size_t q = ((size_t)t * nr + nr - 1) * nn + k;
+            if (Hr[q] > 0.0) lHr[q] += w * A_r[(size_t)t * (nr + 1) + nr];
+        } else if (t < nz + nr) {
+            const int i = t - nz;
+            const size_t q = ((size_t)(nz - 1) * nr + i) * nn + k;
+            if (Hz[q] > 0.0) lHz[q] += w * A_z[(size_t)nz * nr + i];
+        } else if (bc_z_bot == BC_OUTFLOW && nz > 1) {
+            const int i = t - nz - nr;
+            const size_t q = (size_t)i * nn + k;
+            if (-Hz[q] > 0.0) lHz[q] -= w * A_z[i];
+        }
+    }
+}
+
+// y[c] += s * dV[c]
+__global__ void k_adj_axpy_cell(double* y, const double* dV, double s, int n)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) y[c] += s * dV[c];
+}
+
+void launch_adj_column(hipStream_t s, const AdjColArgs& a)
+{
+    const int ncell = a.nr * a.nz;
+    hipLaunchKernelGGL(k_adj_column, dim3((ncell + 63) / 64), dim3(64), 0, s, a);
+}
+
+void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a)
+{
+    dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
+    hipLaunchKernelGGL(k_adj_cell, grid, dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL(k_adj_gather, grid, dim3(64, 4), 0, s, a);
+}
+
+void launch_adj_spectrum(hipStream_t s, const double* Hr, const double* Hz, const double* A_r, const double* A_z,
+                         int nn, int nr, int nz, int bc_z_bot, double inv_Lsim, double* S)
+{
+    hipLaunchKernelGGL(k_adj_spectrum, dim3(nn), dim3(256), 0, s, Hr, Hz, A_r, A_z, nn, nr, nz, bc_z_bot, inv_Lsim,
+                       S);
+}
+
+void launch_adj_spectrum_seed(hipStream_t s, const double* Hr, const double* Hz, const double* A_r,
+                              const double* A_z, int nn, int nr, int nz, int bc_z_bot, double inv_Lsim,
+                              const double* v, double* lHr, double* lHz)
+{
+    hipLaunchKernelGGL(k_adj_spectrum_seed, dim3(nn), dim3(256), 0, s, Hr, Hz, A_r, A_z, nn, nr, nz, bc_z_bot,
+                       inv_Lsim, v, lHr, lHz);
+}
+
+void launch_adj_axpy_cell(hipStream_t s, double* y, const double* dV, double a, int n)
+{
+    hipLaunchKernelGGL(k_adj_axpy_cell, dim3((n + 255) / 256), dim3(256), 0, s, y, dV, a, n);
+}
+
+}  // namespace m1

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -348,6 +349,54 @@ void fused_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc
     c->old_stale = true;
     c->fij_stale = true;
     c->krange_valid = true;
+}
+
+// Device buffers of one adjoint run; freed (after the stream drains) on any exit.
+struct AdjMem {
+    m1lya_ctx* c;
+    std::vector<void*> ptrs;
+    explicit AdjMem(m1lya_ctx* c_) : c(c_) {}
+    AdjMem(const AdjMem&) = delete;
+    AdjMem& operator=(const AdjMem&) = delete;
+    double* get(size_t n)
+    {
+        void* p = nullptr;
+        hipchk(c, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)), "hipMalloc (adjoint)");
+        ptrs.push_back(p);
+        return (double*)p;
+    }
+    int* geti(size_t n)
+    {
+        void* p = nullptr;
+        hipchk(c, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(int)), "hipMalloc (adjoint)");
+        ptrs.push_back(p);
+        return (int*)p;
+    }
+    ~AdjMem()
+    {
+        (void)hipStreamSynchronize(c->stream);
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// The fused step with the advection output copied to X1 (J, H_r, H_Z at
+// X1, X1 + n3, X1 + 2 n3).  The active range must already be in c->krange.
+void replay_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc_r_right, int bc_z_bot,
+                 int bc_z_top, double* X1)
+{
+    const double cdt = c_tilde * dt;
+    run_advect(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    const size_t b = c->n3 * sizeof(double);
+    hipchk(c, hipMemcpyAsync(X1, c->J, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+    hipchk(c, hipMemcpyAsync(X1 + c->n3, c->Hr, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+    hipchk(c, hipMemcpyAsync(X1 + 2 * c->n3, c->Hz, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+    ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED);
+    a.c_tilde = c_tilde; a.dt = dt; a.dw = 1.0 / cdt; a.z_center = z_center; a.bc_z_bot = bc_z_bot;
+    Prof p(c, "column");
+    launch_column(c->stream, a, c->col_blocks);
+    p.done();
+    c->old_stale = true;
+    c->fij_stale = true;
 }
 
 template <class F>
@@ -854,6 +903,182 @@ int m1lya_run_steps(m1lya_ctx* c, int n_steps, double c_tilde, double dt, double
                    "history");
         hipchk(c, hipStreamSynchronize(c->stream), "sync");
         check_err_flag(c, "run_steps");
+    });
+}
+
+int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, double z_center, int bc_r_right,
+                      int bc_z_bot, int bc_z_top, const double* mf_seed, const double* spec_seed,
+                      int checkpoint_every, double* objective, double* history, double* grad_state,
+                      double* grad_cell, double* grad_bin)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_r_right, "BC_r_right");
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        check_bc(c, bc_z_top, "BC_Z_top");
+        if (n_steps < 1) fail(c, M1LYA_ERR_ARG, "n_steps < 1");
+        if (checkpoint_every < 0) fail(c, M1LYA_ERR_ARG, "checkpoint_every < 0");
+        if (!(c_tilde > 0.0) || !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde and dt must be > 0");
+        const size_t n3 = c->n3, nc = c->ncell, B3 = 3 * n3;
+        const int nn = c->nn, nr = c->nr, nz = c->nz;
+        // √N checkpointing: states every S steps; a segment of S steps is replayed with its tape
+        int S = checkpoint_every > 0 ? checkpoint_every : (int)std::ceil(std::sqrt((double)n_steps));
+        S = std::max(1, std::min(S, n_steps));
+        const int nck = (n_steps + S - 1) / S;
+        AdjMem mem(c);
+        double* ck = mem.get((size_t)nck * B3);
+        double* tape = mem.get((size_t)2 * S * B3);   // per step of a segment: X_n, then the advection output
+        double* fin = mem.get(B3);
+        int* kr = mem.geti((size_t)2 * n_steps + 2);
+        double* hist = mem.get((size_t)8 * n_steps);
+        double* lam = mem.get(B3);     // cotangent of the state
+        double* lam1 = mem.get(B3);    // cotangent of the advection output
+        double* pre = mem.get(B3);
+        double* scr = mem.get(9 * n3);
+        double* gbin = mem.get(3 * n3);
+        double* gcell = mem.get(5 * nc);
+        double* dv = mem.get(2 * (size_t)nn);
+        hipStream_t st = c->stream;
+        auto zero = [&](double* p, size_t n) {
+            hipchk(c, hipMemsetAsync(p, 0, n * sizeof(double), st), "hipMemsetAsync");
+        };
+        auto copy3_from_state = [&](double* dst) {
+            const size_t b = n3 * sizeof(double);
+            hipchk(c, hipMemcpyAsync(dst, c->J, b, hipMemcpyDeviceToDevice, st), "copy state");
+            hipchk(c, hipMemcpyAsync(dst + n3, c->Hr, b, hipMemcpyDeviceToDevice, st), "copy state");
+            hipchk(c, hipMemcpyAsync(dst + 2 * n3, c->Hz, b, hipMemcpyDeviceToDevice, st), "copy state");
+        };
+        auto copy3_to_state = [&](const double* src) {
+            const size_t b = n3 * sizeof(double);
+            hipchk(c, hipMemcpyAsync(c->J, src, b, hipMemcpyDeviceToDevice, st), "restore state");
+            hipchk(c, hipMemcpyAsync(c->Hr, src + n3, b, hipMemcpyDeviceToDevice, st), "restore state");
+            hipchk(c, hipMemcpyAsync(c->Hz, src + 2 * n3, b, hipMemcpyDeviceToDevice, st), "restore state");
+        };
+        zero(lam, B3);
+        zero(gbin, 3 * n3);
+        zero(gcell, 5 * nc);
+        zero(c->drhou_r, nc);
+        zero(c->drhou_z, nc);
+
+        // ---- forward sweep: checkpoints, active ranges, scalars ----
+        if (!c->krange_valid) compute_krange(c);
+        for (int s = 0; s < n_steps; ++s) {
+            if (s % S == 0) copy3_from_state(ck + (size_t)(s / S) * B3);
+            hipchk(c, hipMemcpyAsync(kr + 2 * s, c->krange, 2 * sizeof(int), hipMemcpyDeviceToDevice, st), "krange");
+            fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, hist + (size_t)8 * s);
+        }
+        copy3_from_state(fin);
+        hipchk(c, hipMemcpyAsync(kr + 2 * n_steps, c->krange, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
+               "krange");
+        std::vector<double> hh((size_t)8 * n_steps);
+        hipchk(c, hipMemcpyAsync(hh.data(), hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+               "history");
+        check_err_flag(c, "adjoint forward sweep");
+
+        // L_sum = 4π Σ ΔV js; the history's L_Lyα doubles it for a reflecting bottom
+        const double Lsum = hh[3] / (bc_z_bot == M1LYA_REFLECT ? 2.0 : 1.0);
+        const double mf_scale = Lsum > 0.0 ? kC / Lsum : 0.0;
+        double obj = 0.0;
+        if (mf_seed)
+            for (int s = 0; s < n_steps; ++s)
+                for (int q = 0; q < 3; ++q) obj += mf_seed[3 * s + q] * hh[(size_t)8 * s + q];
+        if (spec_seed && Lsum > 0.0) {
+            std::vector<double> Sk(nn);
+            hipchk(c, hipMemcpyAsync(dv, spec_seed, nn * sizeof(double), hipMemcpyHostToDevice, st), "seed");
+            launch_adj_spectrum(st, fin + n3, fin + 2 * n3, c->A_r, c->A_z, nn, nr, nz, bc_z_bot, 1.0 / Lsum,
+                                dv + nn);
+            hipchk(c, hipGetLastError(), "spectrum");
+            hipchk(c, hipMemcpyAsync(Sk.data(), dv + nn, nn * sizeof(double), hipMemcpyDeviceToHost, st), "S");
+            hipchk(c, hipStreamSynchronize(st), "sync");
+            for (int k = 0; k < nn; ++k) obj += spec_seed[k] * Sk[k];
+            launch_adj_spectrum_seed(st, fin + n3, fin + 2 * n3, c->A_r, c->A_z, nn, nr, nz, bc_z_bot, 1.0 / Lsum,
+                                     dv, lam + n3, lam + 2 * n3);
+            hipchk(c, hipGetLastError(), "spectrum seed");
+        }
+
+        // ---- reverse sweep, segment by segment ----
+        for (int m = nck - 1; m >= 0; --m) {
+            const int s0 = m * S, s1 = std::min(s0 + S, n_steps);
+            copy3_to_state(ck + (size_t)m * B3);
+            for (int s = s0; s < s1; ++s) {
+                double* Xn = tape + (size_t)2 * (s - s0) * B3;
+                copy3_from_state(Xn);
+                hipchk(c, hipMemcpyAsync(c->krange, kr + 2 * s, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
+                       "krange");
+                replay_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, Xn + B3);
+            }
+            for (int s = s1 - 1; s >= s0; --s) {
+                const double* Xn = tape + (size_t)2 * (s - s0) * B3;
+                const double* X1 = Xn + B3;
+                const double* Xn1 = (s + 1 < s1) ? tape + (size_t)2 * (s + 1 - s0) * B3
+                                                 : (s1 == n_steps ? fin : ck + (size_t)(m + 1) * B3);
+                AdjColArgs ca;
+                std::memset(&ca, 0, sizeof(ca));
+                ca.nn = nn; ca.nr = nr; ca.nz = nz; ca.recoil = c->recoil;
+                ca.J1 = X1; ca.Hr1 = X1 + n3; ca.Hz1 = X1 + 2 * n3;
+                ca.J4 = Xn1;
+                ca.lJ4 = lam; ca.lHr4 = lam + n3; ca.lHz4 = lam + 2 * n3;
+                ca.lJ1 = lam1; ca.lHr1 = lam1 + n3; ca.lHz1 = lam1 + 2 * n3;
+                ca.X2J = scr; ca.X2Hr = scr + n3; ca.X2Hz = scr + 2 * n3;
+                ca.rowA = scr + 3 * n3; ca.rowB = scr + 4 * n3; ca.rowC = scr + 5 * n3;
+                ca.xs = scr + 6 * n3; ca.mu = scr + 7 * n3; ca.cp = scr + 8 * n3;
+                ca.Hv = c->Hv; ca.phi = c->phi; ca.aF = c->aF;
+                ca.alpha = c->alpha; ca.alpha_c = c->alpha_c; ca.pd = c->pd; ca.js = c->js; ca.T = c->T;
+                ca.dnuD = c->dnuD;
+                ca.dur_dr = c->dur_dr; ca.dur_dz = c->dur_dz; ca.duz_dr = c->duz_dr; ca.duz_dz = c->duz_dz;
+                ca.ur_over_r = c->ur_over_r;
+                ca.nu = c->nu; ca.nuC = c->nuC; ca.dnu = c->dnu; ca.inv_dnu = c->inv_dnu; ca.inv_nuC = c->inv_nuC;
+                ca.dV = c->dV; ca.rcom = c->rcom; ca.zcom = c->zcom;
+                ca.c_tilde = c_tilde; ca.dt = dt; ca.dw = 1.0 / (c_tilde * dt);
+                ca.alpha_floor = c->alpha_floor; ca.z_center = z_center;
+                ca.mf_scale = mf_scale;
+                if (mf_seed) { ca.w0 = mf_seed[3 * s]; ca.w1 = mf_seed[3 * s + 1]; ca.w2 = mf_seed[3 * s + 2]; }
+                ca.g_aF = gbin; ca.g_Hv = gbin + n3; ca.g_phi = gbin + 2 * n3;
+                ca.g_alpha = gcell; ca.g_alpha_c = gcell + nc; ca.g_js = gcell + 2 * nc;
+                ca.g_dnuD = gcell + 3 * nc; ca.g_T = gcell + 4 * nc;
+                {
+                    Prof p(c, "adj_column");
+                    launch_adj_column(st, ca);
+                    p.done();
+                }
+                AdjAdvArgs aa;
+                aa.nn = nn;
+                aa.g = DevGeom{nr, nz, c->A_r, c->A_z, c->inv_dV, c->inv_r_cell, c->ivr, c->ivz};
+                aa.v = DevVel{c->ur_star, c->uz_star, c->dur_dr, c->dur_dz, c->duz_dr, c->duz_dz, c->ur_over_r};
+                aa.J = Xn; aa.Hr = Xn + n3; aa.Hz = Xn + 2 * n3; aa.aF = c->aF;
+                aa.lJ1 = lam1; aa.lHr1 = lam1 + n3; aa.lHz1 = lam1 + 2 * n3;
+                aa.pJ = pre; aa.pHr = pre + n3; aa.pHz = pre + 2 * n3;
+                aa.oJ = lam; aa.oHr = lam + n3; aa.oHz = lam + 2 * n3;
+                aa.gaF = gbin;
+                aa.cdt = c_tilde * dt;
+                aa.bc_r_right = bc_r_right; aa.bc_z_bot = bc_z_bot; aa.bc_z_top = bc_z_top; aa.hll = c->flux;
+                aa.krange = kr + 2 * s;
+                {
+                    Prof p(c, "adj_advect");
+                    launch_adj_advect(st, aa);
+                    p.done();
+                }
+            }
+        }
+        // the context ends in the state after n_steps, as m1lya_run_steps leaves it
+        copy3_to_state(fin);
+        hipchk(c, hipMemcpyAsync(c->krange, kr + 2 * n_steps, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
+               "krange");
+        c->krange_valid = true;
+        c->old_stale = true;
+        c->fij_stale = true;
+        // M_F and S carry 1/L_sum, L_sum = 4π Σ ΔV js
+        if (Lsum > 0.0 && obj != 0.0) launch_adj_axpy_cell(st, gcell + 2 * nc, c->dV, -obj / Lsum * kFourPi, (int)nc);
+        if (objective) *objective = obj;
+        if (history) std::memcpy(history, hh.data(), hh.size() * sizeof(double));
+        if (grad_state)
+            hipchk(c, hipMemcpyAsync(grad_state, lam, B3 * sizeof(double), hipMemcpyDeviceToHost, st), "grad");
+        if (grad_cell)
+            hipchk(c, hipMemcpyAsync(grad_cell, gcell, 5 * nc * sizeof(double), hipMemcpyDeviceToHost, st), "grad");
+        if (grad_bin)
+            hipchk(c, hipMemcpyAsync(grad_bin, gbin, 3 * n3 * sizeof(double), hipMemcpyDeviceToHost, st), "grad");
+        hipchk(c, hipStreamSynchronize(st), "sync");
+        check_err_flag(c, "adjoint");
     });
 }
 

@@ -124,6 +124,95 @@ struct MfArgs {
     double* part;         // [nblocks][8]
 };
 
+// ---- reverse sweep of the fused step (m1lya_adjoint.hip) ----
+struct AdjAdvArgs {
+    int nn;
+    DevGeom g;
+    DevVel v;
+    const double* J;      // advection input X_n
+    const double* Hr;
+    const double* Hz;
+    const double* aF;
+    const double* lJ1;    // cotangent of the advection output
+    const double* lHr1;
+    const double* lHz1;
+    double* pJ;           // cotangent of each cell's flux-divergence sums
+    double* pHr;
+    double* pHz;
+    double* oJ;           // cotangent of X_n
+    double* oHr;
+    double* oHz;
+    double* gaF;          // += d/dα_F
+    double cdt;
+    int bc_r_right, bc_z_bot, bc_z_top, hll;
+    const int* krange;
+};
+
+struct AdjColArgs {
+    int nn, nr, nz, recoil;
+    const double* J1;     // advection output (input of emission/absorption)
+    const double* Hr1;
+    const double* Hz1;
+    const double* J4;     // step output
+    const double* lJ4;    // cotangent of the step output
+    const double* lHr4;
+    const double* lHz4;
+    double* lJ1;          // cotangent of the advection output
+    double* lHr1;
+    double* lHz1;
+    double* X2J;          // scratch [n3] each
+    double* X2Hr;
+    double* X2Hz;
+    double* rowA;
+    double* rowB;
+    double* rowC;
+    double* xs;
+    double* mu;
+    double* cp;
+    const double* Hv;
+    const double* phi;
+    const double* aF;
+    const double* alpha;
+    const double* alpha_c;
+    const double* pd;
+    const double* js;
+    const double* T;
+    const double* dnuD;
+    const double* dur_dr;
+    const double* dur_dz;
+    const double* duz_dr;
+    const double* duz_dz;
+    const double* ur_over_r;
+    const double* nu;
+    const double* nuC;
+    const double* dnu;
+    const double* inv_dnu;
+    const double* inv_nuC;
+    const double* dV;
+    const double* rcom;
+    const double* zcom;
+    double c_tilde, dt, dw, alpha_floor, z_center;
+    double mf_scale;      // c / L_sum (0 when there is no source)
+    double w0, w1, w2;    // seeds of M_F, M_F_radial_net, M_F_radial_out at this step
+    double* g_aF;         // += [n3]
+    double* g_Hv;
+    double* g_phi;
+    double* g_alpha;      // += [ncell]
+    double* g_alpha_c;
+    double* g_js;
+    double* g_dnuD;
+    double* g_T;
+};
+
+void launch_adj_column(hipStream_t s, const AdjColArgs& a);
+void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a);
+void launch_adj_spectrum(hipStream_t s, const double* Hr, const double* Hz, const double* A_r, const double* A_z,
+                         int nn, int nr, int nz, int bc_z_bot, double inv_Lsim, double* S);
+void launch_adj_spectrum_seed(hipStream_t s, const double* Hr, const double* Hz, const double* A_r,
+                              const double* A_z, int nn, int nr, int nz, int bc_z_bot, double inv_Lsim,
+                              const double* v, double* lHr, double* lHz);
+void launch_adj_axpy_cell(hipStream_t s, double* y, const double* dV, double a, int n);
+
 void launch_active_partial(hipStream_t s, const double* J, const double* dnu, const double* dV, int nn,
                            int ncells, int nblocks, double* part, int pstride);
 void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out);

@@ -62,3 +62,40 @@ def sweep_clouds(n_clouds: int = 64, seed: int = DEFAULT_SEED, lut: VoigtLUT | N
 def updates_per_step(clouds) -> int:
     """Cell-frequency updates in one step of every cloud."""
     return int(sum(c.lya.n_nu * c.lya.n_r * c.lya.n_z for c in clouds))
+
+
+def gaussian_field(n_z: int, n_r: int, slope: float = -11.0 / 3.0, seed: int = DEFAULT_SEED) -> np.ndarray:
+    """Zero-mean, unit-variance Gaussian random field on the (N_Z, N_r) grid with power
+    spectrum P(k) ∝ k^slope (white noise filtered by k^(slope/2) in Fourier space)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kz = np.fft.fftfreq(n_z)[:, None]
+    kr = np.fft.fftfreq(n_r)[None, :]
+    k = np.hypot(kz, kr)
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** (0.5 * slope)
+    g = np.fft.ifft2(np.fft.fft2(rng.normal(size=(n_z, n_r))) * amp).real
+    return (g - g.mean()) / g.std()
+
+
+def lognormal_density(n_z: int, n_r: int, sigma_ln: float, slope: float = -11.0 / 3.0,
+                      seed: int = DEFAULT_SEED) -> np.ndarray:
+    """ρ/ρ̄ with ln ρ a Gaussian random field of standard deviation sigma_ln and P(k) ∝ k^slope,
+    normalised to unit mean over the grid."""
+    rho = np.exp(sigma_ln * gaussian_field(n_z, n_r, slope, seed))
+    return rho / rho.mean()
+
+
+def expanding_cloud(n_r: int = 128, n_z: int = 128, n_nu: int = 128, tau0: float = 1e7, T0: float = 1e4,
+                    u_max: float = 200.0, sigma_ln: float = 0.5, seed: int = DEFAULT_SEED,
+                    lut: VoigtLUT | None = None) -> StaticCloud:
+    """Expanding cloud: a point-source sphere of line-centre optical depth τ0 in a Hubble flow of
+    edge speed u_max km/s (GetHubbleFlow, src/StaticCloudTest.jl:34-59), its H I density
+    multiplied by a lognormal field (σ_lnρ = sigma_ln, P(k) ∝ k^-11/3, unit mean)."""
+    from .static_cloud import point_source_cloud
+    cloud = point_source_cloud(n_r, n_z, n_nu, tau0, T0, umax=u_max, lut=lut)
+    rho = lognormal_density(n_z, n_r, sigma_ln, seed=seed)
+    L = cloud.lya
+    L["alpha"] = L["alpha"] * rho
+    L["alpha_F"] = L["alpha"][:, :, None] * L["Hvoigt"]
+    cloud.params.update(sigma_ln=sigma_ln, seed=seed, u_max=u_max)
+    return cloud
