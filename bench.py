@@ -60,8 +60,9 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
-    p.add_argument("--workload", choices=("cone", "sweep"), default="cone",
-                   help="cone: the 256x256x128 cone cloud; sweep: the seeded batch of 64 uneven clouds")
+    p.add_argument("--workload", choices=("cone", "sweep", "adjoint"), default="cone",
+                   help="cone: the 256x256x128 cone cloud; sweep: the seeded batch of 64 uneven clouds; "
+                        "adjoint: gradient of M_F over the steps of the 128x128x128 expanding cloud")
     p.add_argument("--clouds", type=int, default=64, help="sweep: number of clouds")
     p.add_argument("--n-r", type=int, default=256)
     p.add_argument("--n-z", type=int, default=256)
@@ -285,6 +286,61 @@ def main_sweep(args):
     print(json.dumps(line))
 
 
+ADJ_WORKLOAD = ("expanding cloud 128x128 r-Z, 128 bins, tau0=1e7, T=1e4 K, Hubble outflow u_max=200 km/s, "
+                "lognormal density (sigma_ln=0.5, P(k)~k^-11/3), point source; gradient of the per-step M_F "
+                "over {n} FP64 steps, states checkpointed every ceil(sqrt(n)) steps")
+
+
+def main_adjoint(args):
+    """Forward FP64 steps vs the gradient run (forward sweep + segment replays + reverse sweep)."""
+    from lydion_amd import adjoint as A
+    from lydion_amd import m1lya as M
+    from lydion_amd import workloads as W
+    from lydion_amd.static_cloud import developed_state
+
+    cloud = developed_state(W.expanding_cloud(128, 128, 128, tau0=1e7, u_max=200.0, sigma_ln=0.5))
+    n = args.steps
+    upd = cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
+    run = dict(bc_z_bot=cloud.bc_z_bot)
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    M.run_steps(ctx, args.warmup, cloud.c_tilde, cloud.dt, cloud.Zc, history=False, **run)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    M.run_steps(ctx, n, cloud.c_tilde, cloud.dt, cloud.Zc, history=False, **run)
+    ctx.synchronize()
+    fwd = time.perf_counter() - t0
+    ctx.close()
+
+    mf = np.zeros((n, 3))
+    mf[:, 0] = 1.0                 # Σ_s M_F(s)
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    A.adjoint_run(ctx, 2, cloud.c_tilde, cloud.dt, cloud.Zc, mf[:2], **run)   # warm-up
+    ctx.reset_kernel_stats()
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res = A.adjoint_run(ctx, n, cloud.c_tilde, cloud.dt, cloud.Zc, mf, **run)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    ctx.close()
+    if not all(np.all(np.isfinite(g)) for g in res.grad_state.values()):
+        raise RuntimeError("non-finite gradient")
+    line = {
+        "metric": METRIC, "value": upd * n / wall, "unit": "cell-freq updates/s (gradient run)", "n_gpus": 1,
+        "steps": n, "warmup": args.warmup, "ms_per_step": 1e3 * wall / n, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lognormal density, developed radiation field)",
+        "config": {"workload": ADJ_WORKLOAD.format(n=n), "checkpoint_every": int(np.ceil(np.sqrt(n)))},
+        "forward_ms_per_step": 1e3 * fwd / n,
+        "gradient_over_forward": wall / fwd,
+        "kernels_ms_per_step": {k: round(v[1] / n, 4) for k, v in sorted(stats.items())},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -292,6 +348,9 @@ def main():
               "(multi-GPU sharding of a cloud is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "adjoint":
+        main_adjoint(args)
+        return
     if args.workload == "sweep":
         if args.precision != "fp32":
             print("bench.py: the sweep workload measures the FP32 batch (with the FP64 sequential runs beside it)",

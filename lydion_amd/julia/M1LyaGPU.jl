@@ -18,7 +18,7 @@ module M1LyaGPU
 
 export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
-       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!
+       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!
 
 const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
 
@@ -264,6 +264,39 @@ function run_batch!(b::Ptr{Cvoid}, nsteps::Integer)
     hist = Array{Float64}(undef, 8, n, nsteps)
     bcheck(ccall((:m1lya_batch_run_steps, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), b, nsteps, hist), b)
     return hist
+end
+
+# --- reverse-mode gradient of n fused steps (m1lya_adjoint_run) ---
+
+"""
+    adjoint_run!(Lyα, nsteps, c̃, Δt, Z_center; BC_Z_bot=:Reflect, mf_seed=nothing, spec_seed=nothing,
+                 checkpoint_every=0)
+
+Runs `nsteps` fused steps of the attached LyαArrays (as `m1lya_run_steps`) and returns
+`(objective, history, grad_state, grad_cell, grad_bin)` for
+Obj = Σ_s mf_seed[:, s] ⋅ (M_F, M_F_radial_net, M_F_radial_out)(s) + spec_seed ⋅ S(final), with
+S the escaping spectrum (f_esc = Σ_k Δν_k S_k).  grad_state is [N_ν, N_r, N_Z, 3] (J, H_r, H_Z of
+the initial state), grad_cell [N_r, N_Z, 5] (α_Lyα, α_c, ∫js, ΔνD, T-direct), grad_bin
+[N_ν, N_r, N_Z, 3] (α_F, Hvoigt, ϕ_ν).  A ChainRules `rrule` around the step loop is this call.
+"""
+function adjoint_run!(Lyα, nsteps::Integer, c̃::Float64, Δt::Float64, Z_center::Float64;
+                      BC_Z_bot::Symbol=:Reflect, mf_seed=nothing, spec_seed=nothing, checkpoint_every::Integer=0)
+    ctx = ctxof(Lyα)
+    N_ν, N_r, N_Z = size(Lyα.J_Lyα)
+    mf = mf_seed === nothing ? C_NULL : convert(Matrix{Float64}, mf_seed)      # [3, nsteps]
+    sp = spec_seed === nothing ? C_NULL : convert(Vector{Float64}, spec_seed)  # [N_ν]
+    obj = Ref{Float64}(0.0)
+    hist = Array{Float64}(undef, 8, nsteps)
+    gs = Array{Float64}(undef, N_ν, N_r, N_Z, 3)
+    gc = Array{Float64}(undef, N_r, N_Z, 5)
+    gb = Array{Float64}(undef, N_ν, N_r, N_Z, 3)
+    GC.@preserve mf sp check(ccall((:m1lya_adjoint_run, libm1lya), Cint,
+        (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Cint,
+         Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        ctx, nsteps, c̃, Δt, Z_center, bc(:Outflow), bc(BC_Z_bot), bc(:Outflow),
+        mf === C_NULL ? Ptr{Float64}(C_NULL) : pointer(mf), sp === C_NULL ? Ptr{Float64}(C_NULL) : pointer(sp),
+        checkpoint_every, obj, hist, gs, gc, gb), ctx)
+    return obj[], hist, gs, gc, gb
 end
 
 end # module
