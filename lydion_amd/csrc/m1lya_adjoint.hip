@@ -10,11 +10,12 @@
 // three kernels produce the cotangent of X_n and accumulate those of the
 // coefficient fields:
 //
-//   k_adj_column    one thread per cell: E and C in reverse.  Recomputes E(X1)
-//                   and the frequency-coupling rows, solves the transposed
-//                   tridiagonal system, contracts the row cotangents with the
-//                   local derivatives of each row (dual-number passes over its
-//                   inputs), and reverses the flux correction and emission.
+//   k_adj_column    one wavefront per cell, lanes over bins: E and C in
+//                   reverse.  Recomputes E(X1) and the frequency-coupling
+//                   rows, solves the transposed tridiagonal system, contracts
+//                   the row cotangents with the local derivatives of each row
+//                   (dual-number passes over its inputs), and reverses the
+//                   flux correction and emission.
 //   k_adj_cell      one thread per (bin, cell): the cell update of A in
 //                   reverse (floors, causality limit, aberration, D:∇u and
 //                   K_ϕϕ terms); writes the cotangent of each cell's
@@ -293,31 +294,53 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjAdvArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Emission/absorption and the frequency update in reverse, one thread per cell.
+// Emission/absorption and the frequency update in reverse: one wavefront per
+// cell, lanes over bins; the two tridiagonal sweeps (the forward system for x,
+// its transpose for mu) run on lane 0, interleaved.
 // ---------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+}  // namespace
+
 __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
 {
+    extern __shared__ double lds[];
     const int nn = a.nn, nr = a.nr;
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= nr * a.nz) return;
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
     const int i = c % nr, j = c / nr;
     const size_t base = (size_t)c * nn;
+    double* X2J = lds;
+    double* X2r = lds + nn;
+    double* X2z = lds + 2 * nn;
+    double* rA = lds + 3 * nn;
+    double* rB = lds + 4 * nn;
+    double* rC = lds + 5 * nn;
+    double* x = lds + 6 * nn;
+    double* mu = lds + 7 * nn;
+    double* cp = lds + 8 * nn;
+    double* cq = lds + 9 * nn;
+    double* lJ = lds + 10 * nn;    // cotangent of E's output, then of the advection output
+    double* lHr = lds + 11 * nn;
+    double* lHz = lds + 12 * nn;
+    double* gm = lds + 13 * nn;    // row k's derivative contributions to Hvoigt at k-1, k, k+1
+    double* g0 = lds + 14 * nn;
+    double* gp = lds + 15 * nn;
     const double ct = a.c_tilde, dt = a.dt, dw = a.dw;
     const double fpc_ict = (kFourPi / kC) * (1.0 / ct);
     const double ac = a.alpha_c[c], al = a.alpha[c], pd = a.pd[c], jsv = a.js[c];
     const double* Hv = a.Hv + base;
     const double* phi = a.phi + base;
     const double* aF = a.aF + base;
-    double* X2J = a.X2J + base;
-    double* X2r = a.X2Hr + base;
-    double* X2z = a.X2Hz + base;
-    double* lJ = a.lJ1 + base;    // holds the cotangent of E's output until the emission reverse
-    double* lHr = a.lHr1 + base;
-    double* lHz = a.lHz1 + base;
 
     // ---- E forward (values) and the momentum deposit Δρu ----
     double sr = 0.0, sz = 0.0;
-    for (int k = 0; k < nn; ++k) {
+    for (int k = lane; k < nn; k += 64) {
         const double aJ = ac + al * pd * Hv[k];
         const double dJdt = jsv * ct * phi[k];
         X2J[k] = (a.J1[base + k] + dJdt * dt) / (1.0 + aJ * ct * dt);
@@ -327,7 +350,10 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         X2z[k] = h1z / denH;
         sr += a.dnu[k] * (h1r - X2r[k]);
         sz += a.dnu[k] * (h1z - X2z[k]);
+        gm[k] = 0.0; g0[k] = 0.0; gp[k] = 0.0;
     }
+    sr = wsum(sr);
+    sz = wsum(sz);
     // ---- cotangent of Δρu from the seeds of M_F, M_F_radial_net, M_F_radial_out ----
     double lDr = 0.0, lDz = 0.0;
     if (a.mf_scale != 0.0) {
@@ -350,12 +376,6 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
     }
 
     const int n = nn - 2;
-    double* rA = a.rowA + base;
-    double* rB = a.rowB + base;
-    double* rC = a.rowC + base;
-    double* x = a.xs + base;
-    double* mu = a.mu + base;
-    double* cp = a.cp + base;
     const double dnuD = a.dnuD[c];
     const double amax = dmax(al, a.alpha_floor);
     const double PREFACT = 0.5 * amax * (dnuD * dnuD);
@@ -363,7 +383,6 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
     const double RECOIL_0 = kHoverKB / Tc;
     const double durdr = a.dur_dr[c], durdz = a.dur_dz[c], duzdr = a.duz_dr[c], duzdz = a.duz_dz[c];
     const double urr = a.ur_over_r[c];
-
     auto row_const = [&](int k) {
         m1a::RowConst rc;
         rc.nuCk = a.nuC[k]; rc.nuCkp = a.nuC[k + 1]; rc.nuCkm = a.nuC[k - 1];
@@ -377,9 +396,10 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         rc.recoil = a.recoil;
         return rc;
     };
+    __syncthreads();
 
-    // ---- C forward: rows of the coupling matrix and the solution of the solve ----
-    for (int k = 1; k <= nn - 2; ++k) {
+    // ---- C forward: the rows of the coupling matrix ----
+    for (int k = lane + 1; k <= nn - 2; k += 64) {
         Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
         const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
         m1a::RowT<double> row = m1a::tfreq_row<double>(row_const(k), Hv[k - 1], Hv[k], Hv[k + 1], PREFACT,
@@ -388,20 +408,8 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         rB[k - 1] = -(row.Fk + dw);
         rC[k - 1] = row.Fkp;
     }
-    if (n >= 1) {
-        // Thomas on the M-matrix (the forward pivoted gtsv solves the same system)
-        cp[0] = rC[0] / rB[0];
-        x[0] = (-dw * X2J[1]) / rB[0];
-        for (int r = 1; r < n; ++r) {
-            const double m = rB[r] - rA[r] * cp[r - 1];
-            cp[r] = rC[r] / m;
-            x[r] = ((-dw * X2J[r + 1]) - rA[r] * x[r - 1]) / m;
-        }
-        for (int r = n - 2; r >= 0; --r) x[r] = x[r] - cp[r] * x[r + 1];
-    }
-
     // ---- flux correction in reverse (src/UpdateLyaJandH.jl:335-379) ----
-    for (int k = 0; k < nn; ++k) {
+    for (int k = lane; k < nn; k += 64) {
         const double J4 = a.J4[base + k];
         const double J2 = X2J[k], Hr2 = X2r[k], Hz2 = X2z[k];
         const double inv_J2 = 1.0 / dmax(J2, kJeps);
@@ -429,90 +437,112 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         lHr[k] = ratio * lhr;
         lHz[k] = ratio * lhz;
     }
+    __syncthreads();
 
-    // ---- transposed tridiagonal solve: M^T mu = lJ3 ----
-    double lPREF = 0.0, lREC = 0.0;
-    if (n >= 1) {
-        // M^T: sub[r] = C[r-1], diag[r] = B[r], sup[r] = A[r+1]
-        cp[0] = (n > 1 ? rA[1] : 0.0) / rB[0];
-        mu[0] = mu[0] / rB[0];
+    // ---- the forward system M x = -dw J2 (Thomas; M is an M-matrix) and M^T mu = lJ3 ----
+    if (lane == 0 && n >= 1) {
+        double cpr = rC[0] / rB[0], xr = (-dw * X2J[1]) / rB[0];
+        double cqr = (n > 1 ? rA[1] : 0.0) / rB[0], mur = mu[0] / rB[0];
+        cp[0] = cpr; x[0] = xr; cq[0] = cqr; mu[0] = mur;
         for (int r = 1; r < n; ++r) {
-            const double m = rB[r] - rC[r - 1] * cp[r - 1];
-            cp[r] = (r + 1 < n ? rA[r + 1] : 0.0) / m;
-            mu[r] = (mu[r] - rC[r - 1] * mu[r - 1]) / m;
+            const double m = rB[r] - rA[r] * cpr;
+            const double mt = rB[r] - rC[r - 1] * cqr;
+            cpr = rC[r] / m;
+            xr = ((-dw * X2J[r + 1]) - rA[r] * xr) / m;
+            cqr = (r + 1 < n ? rA[r + 1] : 0.0) / mt;
+            mur = (mu[r] - rC[r - 1] * mur) / mt;
+            cp[r] = cpr; x[r] = xr; cq[r] = cqr; mu[r] = mur;
         }
-        for (int r = n - 2; r >= 0; --r) mu[r] = mu[r] - cp[r] * mu[r + 1];
+        for (int r = n - 2; r >= 0; --r) {
+            xr = x[r] - cp[r] * xr;
+            mur = mu[r] - cq[r] * mur;
+            x[r] = xr;
+            mu[r] = mur;
+        }
+    }
+    __syncthreads();
 
-        for (int k = 1; k <= nn - 2; ++k) {
-            const int r = k - 1;
-            const double m_r = mu[r];
-            lJ[k] += -dw * m_r;   // b = -dw J2
-            // cotangents of the matrix entries: dM = -mu x^T on the band
-            const double lFkm = (r >= 1) ? -m_r * x[r - 1] : 0.0;
-            const double lFk = m_r * x[r];                       // B = -(Fk + dw)
-            const double lFkp = (r <= n - 2) ? -m_r * x[r + 1] : 0.0;
-            if (lFkm == 0.0 && lFk == 0.0 && lFkp == 0.0) continue;
-            // local derivatives of the row over its inputs
-            Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
-            const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
-            const m1a::RowConst rc = row_const(k);
-            double g[6];
+    // ---- matrix cotangents contracted with each row's local derivatives ----
+    double lPREF = 0.0, lREC = 0.0;
+    for (int k = lane + 1; k <= nn - 2; k += 64) {
+        const int r = k - 1;
+        const double m_r = mu[r];
+        lJ[k] += -dw * m_r;   // b = -dw J2
+        const double lFkm = (r >= 1) ? -m_r * x[r - 1] : 0.0;
+        const double lFk = m_r * x[r];                       // B = -(Fk + dw)
+        const double lFkp = (r <= n - 2) ? -m_r * x[r + 1] : 0.0;
+        if (lFkm == 0.0 && lFk == 0.0 && lFkp == 0.0) continue;
+        Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
+        const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
+        const m1a::RowConst rc = row_const(k);
+        double g[6];
 #pragma unroll 1
-            for (int d = 0; d < 6; ++d) {
-                if (d == 4 && !a.recoil) { g[d] = 0.0; continue; }
-                Dl hm(Hv[k - 1], d == 0 ? 1.0 : 0.0), h0(Hv[k], d == 1 ? 1.0 : 0.0), hp(Hv[k + 1], d == 2 ? 1.0 : 0.0);
-                Dl pf(PREFACT, d == 3 ? 1.0 : 0.0), rec(RECOIL_0, d == 4 ? 1.0 : 0.0), vel(VEL, d == 5 ? 1.0 : 0.0);
-                m1a::RowT<Dl> row = m1a::tfreq_row<Dl>(rc, hm, h0, hp, pf, rec, vel);
-                g[d] = lFkp * row.Fkp.d + lFk * row.Fk.d + lFkm * row.Fkm.d;
-            }
-            a.g_Hv[base + k - 1] += g[0];
-            a.g_Hv[base + k] += g[1];
-            a.g_Hv[base + k + 1] += g[2];
-            lPREF += g[3];
-            lREC += g[4];
-            const double lDdu = g[5] * kInvC;
-            if (lDdu != 0.0) {
+        for (int d = 0; d < 6; ++d) {
+            if (d == 4 && !a.recoil) { g[d] = 0.0; continue; }
+            Dl hm(Hv[k - 1], d == 0 ? 1.0 : 0.0), h0(Hv[k], d == 1 ? 1.0 : 0.0), hp(Hv[k + 1], d == 2 ? 1.0 : 0.0);
+            Dl pf(PREFACT, d == 3 ? 1.0 : 0.0), rec(RECOIL_0, d == 4 ? 1.0 : 0.0), vel(VEL, d == 5 ? 1.0 : 0.0);
+            m1a::RowT<Dl> row = m1a::tfreq_row<Dl>(rc, hm, h0, hp, pf, rec, vel);
+            g[d] = lFkp * row.Fkp.d + lFk * row.Fk.d + lFkm * row.Fkm.d;
+        }
+        gm[k] = g[0];
+        g0[k] = g[1];
+        gp[k] = g[2];
+        lPREF += g[3];
+        lREC += g[4];
+        const double lDdu = g[5] * kInvC;
+        if (lDdu != 0.0) {
 #pragma unroll 1
-                for (int var = 0; var < 3; ++var) {
-                    Dl tJ(X2J[k], var == 0 ? 1.0 : 0.0), tHr(X2r[k], var == 1 ? 1.0 : 0.0);
-                    Dl tHz(X2z[k], var == 2 ? 1.0 : 0.0);
-                    m1a::EddT<Dl> Dd = m1a::tD_edd(tHr, tHz, tJ);
-                    const double dd = m1a::tD_colon_grad_u(Dd, durdr, durdz, duzdr, duzdz, urr).d;
-                    if (var == 0) lJ[k] += lDdu * dd;
-                    else if (var == 1) lHr[k] += lDdu * dd;
-                    else lHz[k] += lDdu * dd;
-                }
+            for (int var = 0; var < 3; ++var) {
+                Dl tJ(X2J[k], var == 0 ? 1.0 : 0.0), tHr(X2r[k], var == 1 ? 1.0 : 0.0);
+                Dl tHz(X2z[k], var == 2 ? 1.0 : 0.0);
+                m1a::EddT<Dl> Dd = m1a::tD_edd(tHr, tHz, tJ);
+                const double dd = m1a::tD_colon_grad_u(Dd, durdr, durdz, duzdr, duzdz, urr).d;
+                if (var == 0) lJ[k] += lDdu * dd;
+                else if (var == 1) lHr[k] += lDdu * dd;
+                else lHz[k] += lDdu * dd;
             }
         }
     }
+    __syncthreads();
 
     // ---- E in reverse (src/ConstantsAndFunctions.jl:451-573) ----
     double lal = 0.0, lac = 0.0, ljs = 0.0;
-    for (int k = 0; k < nn; ++k) {
+    for (int k = lane; k < nn; k += 64) {
         const double aJ = ac + al * pd * Hv[k];
         const double den = 1.0 + aJ * ct * dt;
         const double lJ2 = lJ[k];
-        lJ[k] = lJ2 / den;
+        a.lJ1[base + k] = lJ2 / den;
         const double q = lJ2 * dt * ct / den;
         a.g_phi[base + k] += q * jsv;
         ljs += q * phi[k];
         const double laJ = -q * X2J[k];
         lac += laJ;
         lal += laJ * pd * Hv[k];
-        a.g_Hv[base + k] += laJ * al * pd;
+        // Hvoigt: emission's α p_d term plus the rows k-1, k, k+1 of the coupling matrix
+        double gh = laJ * al * pd + g0[k];
+        if (k >= 1) gh += gp[k - 1];
+        if (k + 1 < nn) gh += gm[k + 1];
+        a.g_Hv[base + k] += gh;
         const double denH = 1.0 + aF[k] * ct * dt;
         const double dr_k = lDr * fpc_ict * a.dnu[k], dz_k = lDz * fpc_ict * a.dnu[k];
         const double lHr2 = lHr[k], lHz2 = lHz[k];
-        lHr[k] = lHr2 / denH + dr_k * (1.0 - 1.0 / denH);
-        lHz[k] = lHz2 / denH + dz_k * (1.0 - 1.0 / denH);
+        a.lHr1[base + k] = lHr2 / denH + dr_k * (1.0 - 1.0 / denH);
+        a.lHz1[base + k] = lHz2 / denH + dz_k * (1.0 - 1.0 / denH);
         a.g_aF[base + k] += -((lHr2 - dr_k) * X2r[k] + (lHz2 - dz_k) * X2z[k]) * ct * dt / denH;
     }
-    // PREFACT = ½ max(α, α_floor) ΔνD², RECOIL_0 = h/(k_B T)
-    a.g_alpha[c] += lal + ((a.alpha_floor > al) ? 0.0 : lPREF * 0.5 * (dnuD * dnuD));
-    a.g_alpha_c[c] += lac;
-    a.g_js[c] += ljs;
-    a.g_dnuD[c] += lPREF * amax * dnuD;
-    a.g_T[c] += -lREC * kHoverKB / (Tc * Tc);
+    lal = wsum(lal);
+    lac = wsum(lac);
+    ljs = wsum(ljs);
+    lPREF = wsum(lPREF);
+    lREC = wsum(lREC);
+    if (lane == 0) {
+        // PREFACT = ½ max(α, α_floor) ΔνD², RECOIL_0 = h/(k_B T)
+        a.g_alpha[c] += lal + ((a.alpha_floor > al) ? 0.0 : lPREF * 0.5 * (dnuD * dnuD));
+        a.g_alpha_c[c] += lac;
+        a.g_js[c] += ljs;
+        a.g_dnuD[c] += lPREF * amax * dnuD;
+        a.g_T[c] += -lREC * kHoverKB / (Tc * Tc);
+    }
 }
 
 // Escaping luminosity per bin over L_sim (the integrand of f_esc,
@@ -578,7 +608,8 @@ __global__ void k_adj_axpy_cell(double* y, const double* dV, double s, int n)
 void launch_adj_column(hipStream_t s, const AdjColArgs& a)
 {
     const int ncell = a.nr * a.nz;
-    hipLaunchKernelGGL(k_adj_column, dim3((ncell + 63) / 64), dim3(64), 0, s, a);
+    const size_t lds = 16 * (size_t)a.nn * sizeof(double);
+    hipLaunchKernelGGL(k_adj_column, dim3(ncell), dim3(64), lds, s, a);
 }
 
 void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a)

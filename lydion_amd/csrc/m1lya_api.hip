@@ -934,7 +934,6 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
         double* lam = mem.get(B3);     // cotangent of the state
         double* lam1 = mem.get(B3);    // cotangent of the advection output
         double* pre = mem.get(B3);
-        double* scr = mem.get(9 * n3);
         double* gbin = mem.get(3 * n3);
         double* gcell = mem.get(5 * nc);
         double* dv = mem.get(2 * (size_t)nn);
@@ -1019,9 +1018,6 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
                 ca.J4 = Xn1;
                 ca.lJ4 = lam; ca.lHr4 = lam + n3; ca.lHz4 = lam + 2 * n3;
                 ca.lJ1 = lam1; ca.lHr1 = lam1 + n3; ca.lHz1 = lam1 + 2 * n3;
-                ca.X2J = scr; ca.X2Hr = scr + n3; ca.X2Hz = scr + 2 * n3;
-                ca.rowA = scr + 3 * n3; ca.rowB = scr + 4 * n3; ca.rowC = scr + 5 * n3;
-                ca.xs = scr + 6 * n3; ca.mu = scr + 7 * n3; ca.cp = scr + 8 * n3;
                 ca.Hv = c->Hv; ca.phi = c->phi; ca.aF = c->aF;
                 ca.alpha = c->alpha; ca.alpha_c = c->alpha_c; ca.pd = c->pd; ca.js = c->js; ca.T = c->T;
                 ca.dnuD = c->dnuD;

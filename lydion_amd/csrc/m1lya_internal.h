@@ -160,15 +160,6 @@ struct AdjColArgs {
     double* lJ1;          // cotangent of the advection output
     double* lHr1;
     double* lHz1;
-    double* X2J;          // scratch [n3] each
-    double* X2Hr;
-    double* X2Hz;
-    double* rowA;
-    double* rowB;
-    double* rowC;
-    double* xs;
-    double* mu;
-    double* cp;
     const double* Hv;
     const double* phi;
     const double* aF;
