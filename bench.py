@@ -263,6 +263,8 @@ def main_sweep(args):
         err_mf = max(err_mf, rel(h32[:, idx, 0], h64[:, 0]))
         err_fesc = max(err_fesc, rel(h32[:, idx, 4], h64[:, 4]))
 
+    if args.copy_calibration:
+        copy_calibration()
     step_gbs = STEP_ALGO_BYTES["fp32"] * value / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline_sweep(clouds, args.cpu_seconds)
     line = {
