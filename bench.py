@@ -60,9 +60,11 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
-    p.add_argument("--workload", choices=("cone", "sweep", "adjoint"), default="cone",
+    p.add_argument("--workload", choices=("cone", "sweep", "adjoint", "rhd"), default="cone",
                    help="cone: the 256x256x128 cone cloud; sweep: the seeded batch of 64 uneven clouds; "
-                        "adjoint: gradient of M_F over the steps of the 128x128x128 expanding cloud")
+                        "adjoint: gradient of M_F over the steps of the 128x128x128 expanding cloud; "
+                        "rhd: 200 FP32 substeps of the 256x512x128 RHD-like state, with the FP64 gradient of the "
+                        "integrated radial force")
     p.add_argument("--clouds", type=int, default=64, help="sweep: number of clouds")
     p.add_argument("--n-r", type=int, default=256)
     p.add_argument("--n-z", type=int, default=256)
@@ -343,6 +345,61 @@ def main_adjoint(args):
     print(json.dumps(line))
 
 
+RHD_WORKLOAD = ("RHD-like state 256x512 r-Z (open at both Z ends), 128 bins: lognormal n_H (mean 1e3 cm^-3, "
+                "sigma_ln=1, P(k)~k^-11/3), T = clip(1e4 K (n/n_min)^-s, 1e2, 1e4) with s={s:.3f}, Mach-2 turbulent "
+                "velocity, dust alpha_c = kappa_abs(Lya) x 0.01 x 0.1 x rho, central point source of 1e5 Lsun; "
+                "{n} substeps")
+
+
+def main_rhd(args):
+    """FP32 batched substeps of the RHD-like state, the FP64 step beside them, and the FP64
+    gradient of the integrated radial Lyα force Σ_s M_F_radial_net(s) w.r.t. n_H and L."""
+    from lydion_amd import adjoint as A
+    from lydion_amd import m1lya as M
+    from lydion_amd import workloads as W
+    from lydion_amd.static_cloud import developed_state
+
+    cloud = developed_state(W.rhd_like_cloud())
+    n = args.steps
+    upd = cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
+    wall, stats, graph_wall, tiles = run_fp32([cloud], n, args.warmup, not args.no_autotune)
+    value = upd * n / wall
+    steps64 = max(5, n // 10)
+    wall64, stats64 = run_fp64(cloud, steps64, 2)
+    acc = accuracy(cloud, args.accuracy_steps)
+
+    mf = np.zeros((n, 3))
+    mf[:, 1] = 1.0                   # Σ_s M_F_radial_net(s): the integrated radial force
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    t0 = time.perf_counter()
+    res = A.adjoint_run(ctx, n, cloud.c_tilde, cloud.dt, cloud.Zc, mf, bc_z_bot=cloud.bc_z_bot)
+    adj_wall = time.perf_counter() - t0
+    ctx.close()
+    g = A.physical_gradients(cloud, res, with_T=False)
+    step_gbs = STEP_ALGO_BYTES["fp32"] * value / 1e9
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-freq updates/s", "n_gpus": 1, "steps": n,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / n, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded turbulent RHD-like state, developed radiation field)",
+        "config": {"workload": RHD_WORKLOAD.format(s=cloud.params["s_T"], n=n), "grid": "256x512", "n_nu": 128,
+                   "precision": "fp32", "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "step_algo_bytes_per_update": STEP_ALGO_BYTES["fp32"]},
+        "roofline": roofline(stats, "k_step", upd, n, "rhd_256x512x128_fp32"),
+        "cpu_baseline": None,
+        "fp64_plain": {"value": upd * steps64 / wall64, "ms_per_step": 1e3 * wall64 / steps64, "steps": steps64},
+        "speedup_vs_fp64": value / (upd * steps64 / wall64),
+        "graph_ms_per_step": 1e3 * graph_wall / n,
+        "tiles": {"seg_len": tiles[0], "waves_per_block": tiles[1]},
+        "accuracy": acc,
+        "adjoint": {"objective": "sum over substeps of M_F_radial_net", "steps": n, "wall_s": adj_wall,
+                    "ms_per_step": 1e3 * adj_wall / n, "dObj_dlnL": g["L"] * cloud.params["L_lya"],
+                    "dObj_dnH_abs_max": float(np.max(np.abs(g["n_H"]))),
+                    "checkpoint_every": int(np.ceil(np.sqrt(n)))},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -352,6 +409,9 @@ def main():
 
     if args.workload == "adjoint":
         main_adjoint(args)
+        return
+    if args.workload == "rhd":
+        main_rhd(args)
         return
     if args.workload == "sweep":
         if args.precision != "fp32":

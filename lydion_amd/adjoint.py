@@ -98,8 +98,16 @@ def density(cloud) -> np.ndarray:
     return cloud.lya["alpha"] / compute_sigma0(cloud.T)
 
 
-def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, rel_step_T: float = 1e-6) -> dict:
-    """d Obj / d n_H (per cell), d Obj / d T (per cell) and d Obj / d L (scalar)."""
+def _dust_follows_density(cloud) -> bool:
+    """True for clouds whose dust absorption α_c is proportional to n_H and part of α_F
+    (the RHD-like state); the static clouds carry a fixed α_c outside α_F."""
+    return "kappa_abs" in cloud.params
+
+
+def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, rel_step_T: float = 1e-6,
+                       with_T: bool = True) -> dict:
+    """d Obj / d n_H (per cell), d Obj / d T (per cell; skipped when with_T is False) and
+    d Obj / d L (scalar)."""
     L = cloud.lya
     T = np.asarray(cloud.T, dtype=np.float64)
     gc, gb = res.grad_cell, res.grad_bin
@@ -108,6 +116,14 @@ def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, r
     alpha = L["alpha"]
     Hv = L["Hvoigt"]
     g_nH = sig * (gc["alpha"] + np.sum(Hv * gb["alpha_F"], axis=-1))
+    if _dust_follows_density(cloud):
+        # α_c ∝ n_H enters α_c and every bin of α_F
+        dust = np.divide(L["alpha_c"], nH, out=np.zeros_like(nH), where=nH > 0)
+        g_nH = g_nH + dust * (gc["alpha_c"] + np.sum(gb["alpha_F"], axis=-1))
+    L_lya = cloud.params["L_lya"]
+    g_L = float(np.sum(gc["js"] * L["js"])) / L_lya      # js ∝ L
+    if not with_T:
+        return {"n_H": g_nH, "T": None, "L": g_L}
     # temperature: σ0 ∝ T^-1/2, ΔνD ∝ T^1/2, Voigt tables by central differences of the setup
     h = rel_step_T * T
     _, Hp, php = setup_tables(cloud, T + h, lut)
@@ -119,9 +135,6 @@ def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, r
     dalphaF = dalpha[..., None] * Hv + alpha[..., None] * dHv
     g_T = (gc["T"] + gc["dnuD"] * ddnuD + gc["alpha"] * dalpha
            + np.sum(gb["alpha_F"] * dalphaF + gb["Hvoigt"] * dHv + gb["phi"] * dphi, axis=-1))
-    # luminosity: js ∝ L
-    L_lya = cloud.params["L_lya"]
-    g_L = float(np.sum(gc["js"] * L["js"])) / L_lya
     return {"n_H": g_nH, "T": g_T, "L": g_L}
 
 
@@ -142,5 +155,10 @@ def cloud_with(cloud, n_H: np.ndarray | None = None, T: np.ndarray | None = None
     alpha = nH * compute_sigma0(T1)
     out["alpha"] = alpha
     out["alpha_F"] = alpha[..., None] * Hv
+    if _dust_follows_density(cloud):
+        nH0 = density(cloud)
+        dust = np.divide(L["alpha_c"], nH0, out=np.zeros_like(nH0), where=nH0 > 0)
+        out["alpha_c"] = dust * nH
+        out["alpha_F"] = out["alpha_F"] + out["alpha_c"][..., None]
     out["js"] = L["js"] * L_scale
     return out

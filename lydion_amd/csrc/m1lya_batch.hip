@@ -151,6 +151,24 @@ __device__ __forceinline__ double wave_sum64(double v)
 // 1/x from the hardware estimate (relative error ~2^-24): drcp0 as is, for values that
 // only set a relaxation rate; drcp1 after one Newton step (~2e-15), for pivots and
 // quantities that set an equilibrium; drcp after two (a few ulp).
+// OneMinusϕ_L_and_χ_L / OneMinusϕ_R_and_χ_R (src/FreqFactors.jl:54-133) in FP64
+__device__ __forceinline__ void omphi_L64(double P, double& o, double& chi)
+{
+    if (P >= kPmax) { o = exp(-kPmax); chi = (1.0 - o) / kPmax; return; }
+    if (P <= -kPmax) { o = exp(kPmax); chi = (1.0 - o) / (-kPmax); return; }
+    if (fabs(P) < kPmin) { o = 1.0 - P + 0.5 * (P * P); chi = 1.0 - 0.5 * P; return; }
+    o = exp(-P);
+    chi = (1.0 - o) / P;
+}
+__device__ __forceinline__ void omphi_R64(double P, double& o, double& chi)
+{
+    if (P >= kPmax) { o = exp(kPmax); chi = (1.0 - o) / kPmax; return; }
+    if (P <= -kPmax) { o = exp(-kPmax); chi = (1.0 - o) / (-kPmax); return; }
+    if (fabs(P) < kPmin) { o = 1.0 + P + 0.5 * (P * P); chi = -1.0 - 0.5 * P; return; }
+    o = exp(P);
+    chi = (1.0 - o) / P;
+}
+
 __device__ __forceinline__ double drcp0(double x) { return __builtin_amdgcn_rcp(x); }
 
 __device__ __forceinline__ double drcp1(double x)
@@ -256,6 +274,44 @@ struct LaneBins {
 // Block = wpb wavefronts, one row segment each; all segments of a block belong
 // to one cloud, whose partial row the block writes.
 // ------------------------------------------------------------------------
+// One row of UpdateFreqFactors! (src/FreqFactors.jl:142-278) as the reference builds it, with
+// bin k's own VEL = D:∇u / c in the exponents of bins k-1, k, k+1 (cells whose velocity
+// gradient has shear or anisotropy).
+__device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const float* __restrict__ Hv, int k, int nn,
+                                       double hr, double hz, double jo, double durdr, double durdz, double duzdr,
+                                       double duzdz, double urr, double PREF, double R0, int recoil, double dw,
+                                       double& ra, double& rb, double& rc, double& rd)
+{
+    double Dr, Dz, Dx;
+    D_edd64(hr, hz, jo, Dr, Dz, Dx);
+    double Dpp = 1.0 - Dr - Dz;
+    Dpp = Dpp > 0.0 ? Dpp : 0.0;
+    const double v = (Dr * durdr + Dx * (durdz + duzdr) + Dz * duzdz + Dpp * urr) * kInvC;
+    auto expo = [&](int x, double half, double& qx) {
+        const double Kx = PREF * (double)Hv[x];
+        qx = half * (1.0 / Kx);
+        double Cx = v * bins[x].nuC;
+        if (recoil) Cx += Kx * (R0 - 2.0 * bins[x].inv_nuC);
+        return fmin(fmax(Cx * qx, -kPmax), kPmax);
+    };
+    double qk, oLk, xLk, oRk, xRk;
+    const double Pk = expo(k, bins[k].half_R, qk);
+    omphi_L64(Pk, oLk, xLk);
+    omphi_R64(Pk, oRk, xRk);
+    const double idn = bins[k].inv_dnu;
+    double qkp, oRkp, xRkp;
+    omphi_R64(expo(k + 1, bins[k + 1].half_L, qkp), oRkp, xRkp);
+    const double den_kp = idn * (1.0 / (xLk * qk - xRkp * qkp));
+    double qkm, oLkm, xLkm;
+    omphi_L64(expo(k - 1, bins[k - 1].half_R, qkm), oLkm, xLkm);
+    const double den_km = idn * (1.0 / (xLkm * qkm - xRk * qk));
+    ra = (k == 1) ? 0.0 : oLkm * den_km;
+    rc = (k == nn - 2) ? 0.0 : oRkp * den_kp;
+    rb = -((oLk * den_kp + oRk * den_km) + dw);
+    rd = -dw * jo;
+}
+
+template <bool kShear>
 __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ clouds, Arrays A,
                                                  const Seg* __restrict__ segs, int nblk, int wpb, int parity,
                                                  const int* __restrict__ krange, double* __restrict__ part)
@@ -545,80 +601,102 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         // of the bin centre, ~1e-12 relative.)
         const bool vel = cl.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
                                         cr.duzdz != 0.0 || cr.urr != 0.0);
+        // With an isotropic velocity gradient (∂u_r/∂r = ∂u_Z/∂Z = u_r/r, no shear) D:∇u is the
+        // trace times the rate for every bin, so VEL is the same in all bins and the shared-G
+        // interface form below equals the reference rows.  Otherwise VEL differs between bins and
+        // each row is built as the reference builds it, with its own bin's VEL in the exponents of
+        // its three neighbours (src/FreqFactors.jl:142-278).
+        const bool aniso =
+            kShear && vel && !((cr.durdz + cr.duzdr) == 0.0 && cr.durdr == cr.duzdz && cr.duzdz == cr.urr);
         const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
         const double R0 = cl.recoil ? kHoverKB / cr.T : 0.0;
-        double Aq[2], oLq[2], Bpq[2], oRpq[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            Aq[q] = 1.0; oLq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
-            if (!L.val[q]) continue;
-            const BinRec bk = bins[L.kb[q]];
-            double VEL = 0.0;
-            if (vel) {
-                double Dr, Dz, Dx;
-                D_edd64(hr[q], hz[q], jo[q], Dr, Dz, Dx);
-                double Dpp = 1.0 - Dr - Dz;
-                Dpp = Dpp > 0.0 ? Dpp : 0.0;
-                VEL = (Dr * cr.durdr + Dx * (cr.durdz + cr.duzdr) + Dz * cr.duzdz + Dpp * cr.urr) * kInvC;
-            }
-            const double K = PREF * hv[q];
-            const double iK = drcp1(K);
-            double C = VEL * bk.nuC;
-            if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
-            const double qR = bk.half_R * iK, qL = bk.half_L * iK;
-            const double P = fmin(fmax(C * qR, -kPmax), kPmax);
-            const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
-            double oL, xL, oRp, xRp;
-            if (fabs(P) < kPmin) {
-                oL = 1.0 - P + 0.5 * (P * P);
-                xL = 1.0 - 0.5 * P;
-            } else {
-                oL = exp(-P);
-                xL = (1.0 - oL) * drcp0(P);
-            }
-            if (fabs(Pp) < kPmin) {
-                oRp = 1.0 + Pp + 0.5 * (Pp * Pp);
-                xRp = -1.0 - 0.5 * Pp;
-            } else {
-                // exp(P') = exp(P)·exp(P' - P); P' - P is the rounding of the bin centre
-                const double d = Pp - P;
-                oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp(Pp);
-                xRp = (1.0 - oRp) * drcp0(Pp);
-            }
-            Aq[q] = xL * qR;
-            oLq[q] = oL;
-            Bpq[q] = xRp * qL;
-            oRpq[q] = oRp;
-        }
         const int R1 = R - 1;
-        // neighbours: bin k-1 -> oL (lane above for q = 0), bin k+1 -> B', oR' (lane below for q = R-1)
-        const double oL_prev = from_prev_lane(R1 ? oLq[1] : oLq[0]);
-        const double Bp_next = from_next_lane(Bpq[0]), oRp_next = from_next_lane(oRpq[0]);
-        // interfaces owned by this lane: Gq[q] = G at the top of bin lR+q
-        double Gq[2];
-        Gq[0] = drcp0(Aq[0] - ((R1 == 0) ? Bp_next : Bpq[1]));
-        Gq[1] = drcp0(Aq[1] - Bp_next);
-        const double G_prev = from_prev_lane(R1 ? Gq[1] : Gq[0]);
         double ra[2], rb[2], rc[2], rd[2];
+        if (!aniso) {
+            double Aq[2], oLq[2], Bpq[2], oRpq[2];
+    #pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                Aq[q] = 1.0; oLq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
+                if (!L.val[q]) continue;
+                const BinRec bk = bins[L.kb[q]];
+                double VEL = 0.0;
+                if (vel) {
+                    double Dr, Dz, Dx;
+                    D_edd64(hr[q], hz[q], jo[q], Dr, Dz, Dx);
+                    double Dpp = 1.0 - Dr - Dz;
+                    Dpp = Dpp > 0.0 ? Dpp : 0.0;
+                    VEL = (Dr * cr.durdr + Dx * (cr.durdz + cr.duzdr) + Dz * cr.duzdz + Dpp * cr.urr) * kInvC;
+                }
+                const double K = PREF * hv[q];
+                const double iK = drcp1(K);
+                double C = VEL * bk.nuC;
+                if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
+                const double qR = bk.half_R * iK, qL = bk.half_L * iK;
+                const double P = fmin(fmax(C * qR, -kPmax), kPmax);
+                const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
+                double oL, xL, oRp, xRp;
+                if (fabs(P) < kPmin) {
+                    oL = 1.0 - P + 0.5 * (P * P);
+                    xL = 1.0 - 0.5 * P;
+                } else {
+                    oL = exp(-P);
+                    xL = (1.0 - oL) * drcp0(P);
+                }
+                if (fabs(Pp) < kPmin) {
+                    oRp = 1.0 + Pp + 0.5 * (Pp * Pp);
+                    xRp = -1.0 - 0.5 * Pp;
+                } else {
+                    // exp(P') = exp(P)·exp(P' - P); P' - P is the rounding of the bin centre
+                    const double d = Pp - P;
+                    oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp(Pp);
+                    xRp = (1.0 - oRp) * drcp0(Pp);
+                }
+                Aq[q] = xL * qR;
+                oLq[q] = oL;
+                Bpq[q] = xRp * qL;
+                oRpq[q] = oRp;
+            }
+            // neighbours: bin k-1 -> oL (lane above for q = 0), bin k+1 -> B', oR' (lane below for q = R-1)
+            const double oL_prev = from_prev_lane(R1 ? oLq[1] : oLq[0]);
+            const double Bp_next = from_next_lane(Bpq[0]), oRp_next = from_next_lane(oRpq[0]);
+            // interfaces owned by this lane: Gq[q] = G at the top of bin lR+q
+            double Gq[2];
+            Gq[0] = drcp0(Aq[0] - ((R1 == 0) ? Bp_next : Bpq[1]));
+            Gq[1] = drcp0(Aq[1] - Bp_next);
+            const double G_prev = from_prev_lane(R1 ? Gq[1] : Gq[0]);
+
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
-            const int k = L.kb[q];
-            if (!L.val[q] || k < 1 || k > nn - 2) continue;
-            const double oLm = (q == 0) ? oL_prev : oLq[0];
-            const double oRpp = (q == R1) ? oRp_next : oRpq[1];
-            const double Gm = (q == 0) ? G_prev : Gq[0];
-            const double Gp = Gq[q];
-            const double idn = bins[k].inv_dnu;
-            const double Fkp = oRpp * Gp * idn;
-            const double Fkm = oLm * Gm * idn;
-            const double Fk = (oLq[q] * Gp + oRpq[q] * Gm) * idn;
-            // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215); gtsv drops
-            // the couplings to bins 0 and N_ν-1, which keep their value
-            ra[q] = (k == 1) ? 0.0 : Fkm;
-            rc[q] = (k == nn - 2) ? 0.0 : Fkp;
-            rb[q] = -(Fk + dw);
-            rd[q] = -dw * jo[q];
+            for (int q = 0; q < 2; ++q) {
+                ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
+                const int k = L.kb[q];
+                if (!L.val[q] || k < 1 || k > nn - 2) continue;
+                const double idn = bins[k].inv_dnu;
+                const double oLm = (q == 0) ? oL_prev : oLq[0];
+                const double oRpp = (q == R1) ? oRp_next : oRpq[1];
+                const double Gm = (q == 0) ? G_prev : Gq[0];
+                const double Gp = Gq[q];
+                const double Fkp = oRpp * Gp * idn;
+                const double Fkm = oLm * Gm * idn;
+                const double Fk = (oLq[q] * Gp + oRpq[q] * Gm) * idn;
+                // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215); gtsv drops
+                // the couplings to bins 0 and N_ν-1, which keep their value
+                ra[q] = (k == 1) ? 0.0 : Fkm;
+                rc[q] = (k == nn - 2) ? 0.0 : Fkp;
+                rb[q] = -(Fk + dw);
+                rd[q] = -dw * jo[q];
+            }
+        } else {
+            // one bin at a time (a scheduling barrier between them keeps the two rows' temporaries
+            // from being live together)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
+                const int k = L.kb[q];
+                if (L.val[q] && k >= 1 && k <= nn - 2)
+                    aniso_row(bins, Hvp + base, k, nn, hr[q], hz[q], jo[q], cr.durdr, cr.durdz, cr.duzdr,
+                              cr.duzdz, cr.urr, PREF, R0, cl.recoil, dw, ra[q], rb[q], rc[q], rd[q]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
 
         // ================= tridiagonal solve: CR level + PCR over the lanes =================
@@ -904,6 +982,7 @@ struct m1lya_batch {
 
     size_t n3 = 0, nc = 0, nfr = 0, nfz = 0, nbin = 0, nivr = 0, nivz = 0;
     std::vector<CellRec> h_rec;            // host mirror of the cell records
+    bool shear = false;                    // a cell's velocity gradient is sheared or anisotropic
     std::vector<BinRec> h_bin;
     CellRec* d_rec = nullptr;
     BinRec* d_bin = nullptr;
@@ -1119,8 +1198,14 @@ void launch_step_kernels(m1lya_batch* b, int parity)
     hipStream_t s = b->stream;
     {
         BProf p(b, "k_step");
-        hipLaunchKernelGGL(k_step, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
-                           parity, b->krange, b->part);
+        // the variant with the reference-row path for sheared / anisotropic velocity gradients only
+        // when a cell of the batch needs it: it costs registers the common path does not pay for
+        if (b->shear)
+            hipLaunchKernelGGL(k_step<true>, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs,
+                               b->nblk, b->wpb, parity, b->krange, b->part);
+        else
+            hipLaunchKernelGGL(k_step<false>, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs,
+                               b->nblk, b->wpb, parity, b->krange, b->part);
         p.done();
     }
     {
@@ -1484,6 +1569,14 @@ int m1lya_batch_upload(m1lya_batch* b, int cloud, int field, const double* host,
         if (field >= M1LYA_DUR_DR && field <= M1LYA_UR_OVER_R) {
             cl.has_vel = 1;
             b->cl_dirty = true;
+            // recompute whether any cell needs the per-row VEL path (see k_step)
+            bool sh = false;
+            for (const CellRec& r : b->h_rec)
+                if ((r.durdz + r.duzdr) != 0.0 || r.durdr != r.duzdz || r.duzdz != r.urr) { sh = true; break; }
+            if (sh != b->shear) {
+                b->shear = sh;
+                drop_graphs(b);
+            }
         }
     });
 }

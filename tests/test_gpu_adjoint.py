@@ -190,7 +190,7 @@ def _per_cell_check(label, cells, grad, fd_of_cell, x0, rel_steps, cloud):
         fds = fd_of_cell(j, i, [r * x0[j, i] for r in rel_steps])
         smooth = abs(fds[0] - fds[1]) <= FD_TOL * floor
         e = abs(fds[1] - ad) / floor
-        print(f"[adjoint fd] u_max={cloud.params['u_max']} {label} cell ({j},{i}) fd {fds[1]:+.8e} "
+        print(f"[adjoint fd] {cloud.params.get('u_max', 'rhd')} {label} cell ({j},{i}) fd {fds[1]:+.8e} "
               f"(larger step {fds[0]:+.8e}) adj {ad:+.8e} rel {e:.1e}{'' if smooth else '  [kink: skipped]'}")
         if smooth:
             used += 1
@@ -249,3 +249,36 @@ def test_expanding_cloud_density_and_temperature_gradients(expanding, lut):
           f"n_H: worst {worst_nH:.1e} over {used_nH} cells; T: worst {worst_T:.1e} over {used_T} cells")
     assert used_nH >= 24 and used_T >= 24, (used_nH, used_T)
     assert worst_nH <= FD_TOL and worst_T <= FD_TOL and eL <= FD_TOL, (worst_nH, worst_T, eL)
+
+
+def test_rhd_like_state_radial_force_gradient(lut):
+    """The integrated radial force Σ_s M_F_radial_net(s) over 8 steps of a turbulent RHD-like
+    state (48x96 r-Z, 64 bins; open Z ends, T tied to density, Mach-2 velocity field, dust ∝ n_H
+    inside α_c and α_F): d/dn_H on 12 random cells and d/d ln L against central differences."""
+    cloud = developed_state(W.rhd_like_cloud(48, 96, 64, lut=lut))
+    n = 8
+    mf = np.zeros((n, 3))
+    mf[:, 1] = 1.0
+    res, ctx = _adjoint(cloud, n, mf, None, checkpoint_every=3)
+    ctx.close()
+    g = A.physical_gradients(cloud, res, lut, with_T=False)
+    nH0 = A.density(cloud)
+    rng = np.random.Generator(np.random.PCG64(23))
+    cells = np.argwhere(np.ones_like(nH0, dtype=bool)[2:-2, 2:-2]) + 2
+    cells = cells[rng.choice(len(cells), size=12, replace=False)]
+
+    def fd_nH(j, i, steps):
+        def f(x):
+            nH = nH0.copy()
+            nH[j, i] = x
+            return _objective(cloud, n, mf, None, A.cloud_with(cloud, n_H=nH, lut=lut))
+        return _fd_pair(f, nH0[j, i], steps)
+
+    worst, used = _per_cell_check("dF/dnH", cells, g["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud)
+    hL = 1e-5
+    fd = (_objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 + hL, lut=lut))
+          - _objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 - hL, lut=lut))) / (2 * hL)
+    ad = g["L"] * cloud.params["L_lya"]
+    eL = abs(fd - ad) / max(abs(fd), 1e-300)
+    print(f"[adjoint fd] rhd dF/dlnL fd {fd:+.8e} adj {ad:+.8e} rel {eL:.1e}; n_H worst {worst:.1e} over {used}")
+    assert used >= 9 and worst <= FD_TOL and eL <= FD_TOL, (worst, used, eL)

@@ -226,3 +226,18 @@ def test_sweep_first_steps_from_rest(sweep):
         _check_history(hb[:, idx, :], h64, f"sweep cloud {idx} from rest", start=5, fesc_min=1e-12)
         ctx.close()
     b.close()
+
+
+def test_rhd_like_state_vs_fp64(lut):
+    """A turbulent RHD-like state (48x96 r-Z, 64 bins: lognormal n_H, T from 1e2 to 1e4 K tied to
+    density, Mach-2 velocity field, dust ∝ n_H, open Z ends) from a developed field: M_F and
+    f_esc of every step of 20 against the FP64 step."""
+    from lydion_amd import workloads as W
+    cloud = developed_state(W.rhd_like_cloud(48, 96, 64, lut=lut))
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(20)
+    h64, ctx = _fp64_history(cloud, 20)
+    _check_history(hb[:, 0, :], h64, "rhd-like")
+    assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
+    ctx.close()
+    b.close()

@@ -37,3 +37,23 @@ def test_sweep_cloud_setup():
         else:
             # js = 3L/((4π)² R³) inside the sphere; the box holds its Z >= 0 half (reflecting bottom)
             assert abs(Lsrc - 0.5) < 0.05
+
+
+def test_rhd_like_state_fields():
+    c = W.rhd_like_cloud(32, 64, 32)
+    L = c.lya
+    assert c.T.min() == 100.0 and c.T.max() == 1e4                      # T spans 1e2..1e4 K
+    nH = L["alpha"] / W_sigma0(c.T)
+    assert np.allclose(nH.mean(), 1e3, rtol=1e-12)                        # unit-mean lognormal x 1e3
+    assert np.log(nH).std() == pytest_approx(1.0, rel=1e-12)
+    u = np.sqrt(np.mean(c.vel["u_r"] ** 2 + c.vel["u_z"] ** 2))
+    assert u == pytest_approx(2.0 * c.params["c_s"], rel=1e-12)          # Mach 2
+    assert np.all(c.vel["ur_star"][:, 0] == 0.0)                          # no flow through the axis
+    dust = L["alpha_c"] / nH
+    assert np.allclose(dust, dust.flat[0], rtol=1e-12)                    # α_c ∝ n_H
+    assert np.allclose(L["alpha_F"], L["alpha"][..., None] * L["Hvoigt"] + L["alpha_c"][..., None], rtol=1e-14)
+    assert np.count_nonzero(L["js"]) == 1 and c.bc_z_bot == 1
+
+
+from pytest import approx as pytest_approx  # noqa: E402
+from lydion_amd.lya_arrays import compute_sigma0 as W_sigma0  # noqa: E402
