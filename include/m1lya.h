@@ -219,6 +219,31 @@ int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
 int m1lya_run_steps(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
                     int bc_r_right, int bc_z_bot, int bc_z_top, double *history);
 
+/* ---- device-side coefficient refresh -------------------------------------- */
+typedef struct {
+    double nu_lya;          /* ν_Lyα [Hz] */
+    double b_prefact;       /* b(T) = b_prefact √T                (Compute_b) */
+    double nu_over_c;       /* ΔνD = b ν_Lyα / c                  (Compute_ΔνD) */
+    double a_prefact;       /* a = a_prefact / b                  (Compute_a) */
+    double sigma0_prefact;  /* σ0 = sigma0_prefact / b            (Compute_σ0) */
+} m1lya_line_constants;
+
+/* The Voigt lookup tables of CreateVoigtLUTs (src/FastInterpolations.jl:26-80): luts is
+ * [5][nx], the node values of erf, dawson, exp(-x²), x exp(-x²) and (1-2x²) exp(-x²) on
+ * x_i = -xmax + i·2 xmax/(nx-1).  Copied to the device once. */
+int m1lya_set_voigt_lut(m1lya_ctx *ctx, const double *luts, int nx, double xmax, const m1lya_line_constants *lc);
+
+/* PrepareLyαSubCycling! (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2090-2155) with
+ * Update_Hvoigt_And_ϕν_cell! (:2039-2082) on the device for every cell: from T and n_HI
+ * [N_Z][N_r] it recomputes ΔνD, Hvoigt (the bin average Htophat of the Voigt profile,
+ * src/ConstantsAndFunctions.jl:265-279), ϕ_ν, α_Lyα = n_HI σ0(T) and α_F = α_Lyα Hvoigt + α_c,F.
+ * The thermochemistry terms are inputs: alpha_c (continuum absorption α_c), alpha_c_F
+ * (continuum flux opacity, added to every bin of α_F), js (∫dν js) and pd; each may be NULL
+ * (α_c, js and pd keep their device values, α_c,F = 0).  phi_all_bins = 1 normalises ϕ_ν over
+ * all bins (the RHD form), 0 over the core bins (Update_ϕν!, src/ConstantsAndFunctions.jl:362-393). */
+int m1lya_prepare_subcycling(m1lya_ctx *ctx, const double *T, const double *n_HI, const double *alpha_c,
+                             const double *alpha_c_F, const double *js, const double *pd, int phi_all_bins);
+
 /* ---- reverse-mode gradient of the fused step ------------------------------
  * The reference has no adjoint; this entry point is what a ChainRules `rrule`
  * around m1lya_run_steps calls.  It runs n_steps fused steps from the
@@ -280,6 +305,13 @@ int  m1lya_batch_get_scale(const m1lya_batch *b, int cloud, double *J0);
 int  m1lya_batch_upload(m1lya_batch *b, int cloud, int field, const double *host, size_t count);
 int  m1lya_batch_download(m1lya_batch *b, int cloud, int field, double *host, size_t count);
 /* c̃, Δt, Z_center and the boundary conditions of one cloud's steps. */
+/* The device-side coefficient refresh of one cloud, as m1lya_set_voigt_lut and
+ * m1lya_prepare_subcycling; the refreshed Hvoigt, ϕ_ν and α_F are stored in FP32. */
+int  m1lya_batch_set_voigt_lut(m1lya_batch *b, const double *luts, int nx, double xmax,
+                               const m1lya_line_constants *lc);
+int  m1lya_batch_prepare_subcycling(m1lya_batch *b, int cloud, const double *T, const double *n_HI,
+                                    const double *alpha_c, const double *alpha_c_F, const double *js,
+                                    const double *pd, int phi_all_bins);
 int  m1lya_batch_set_step(m1lya_batch *b, int cloud, double c_tilde, double dt, double z_center,
                           int bc_r_right, int bc_z_bot, int bc_z_top);
 /* n_steps steps of every cloud.  history (may be NULL) receives

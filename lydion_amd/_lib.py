@@ -51,6 +51,12 @@ class FreqGrid(C.Structure):
     ]
 
 
+class LineConstants(C.Structure):
+    """m1lya_line_constants"""
+    _fields_ = [("nu_lya", C.c_double), ("b_prefact", C.c_double), ("nu_over_c", C.c_double),
+                ("a_prefact", C.c_double), ("sigma0_prefact", C.c_double)]
+
+
 EXPORTED = [
     "m1lya_abi_version", "m1lya_create", "m1lya_destroy", "m1lya_last_error", "m1lya_set_geometry",
     "m1lya_set_freq_grid", "m1lya_upload", "m1lya_download", "m1lya_fill", "m1lya_synchronize",
@@ -58,6 +64,7 @@ EXPORTED = [
     "m1lya_update_freq_factors", "m1lya_frequency_update_all_cells", "m1lya_update_f_mf_l",
     "m1lya_update_f_mf_l_gravity", "m1lya_compute_f_esc", "m1lya_reset_drhou", "m1lya_update_e_lya",
     "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_adjoint_run",
+    "m1lya_set_voigt_lut", "m1lya_prepare_subcycling",
     "m1lya_set_profiling",
     "m1lya_kernel_stats_count", "m1lya_kernel_stats", "m1lya_reset_kernel_stats", "m1lya_selftest_math",
     "m1lya_batch_create", "m1lya_batch_destroy", "m1lya_batch_last_error", "m1lya_batch_count",
@@ -66,6 +73,7 @@ EXPORTED = [
     "m1lya_batch_scalars", "m1lya_batch_active_bins", "m1lya_batch_set_tiles", "m1lya_batch_get_tiles",
     "m1lya_batch_autotune", "m1lya_batch_set_graph", "m1lya_batch_synchronize", "m1lya_batch_set_profiling",
     "m1lya_batch_kernel_stats_count", "m1lya_batch_kernel_stats", "m1lya_batch_reset_kernel_stats",
+    "m1lya_batch_set_voigt_lut", "m1lya_batch_prepare_subcycling",
 ]
 
 _lib = None
@@ -110,6 +118,8 @@ def load() -> C.CDLL:
         "m1lya_step": (i, [vp, d, d, d, i, i, i, _dp]),
         "m1lya_run_steps": (i, [vp, i, d, d, d, i, i, i, _dp]),
         "m1lya_adjoint_run": (i, [vp, i, d, d, d, i, i, i, _dp, _dp, i, _dp, _dp, _dp, _dp, _dp]),
+        "m1lya_set_voigt_lut": (i, [vp, _dp, i, d, C.POINTER(LineConstants)]),
+        "m1lya_prepare_subcycling": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "m1lya_set_profiling": (i, [vp, i]),
         "m1lya_kernel_stats_count": (i, [vp]),
         "m1lya_kernel_stats": (i, [vp, i, C.c_char_p, sz, C.POINTER(C.c_long), _dp]),
@@ -117,6 +127,8 @@ def load() -> C.CDLL:
         "m1lya_selftest_math": (i, [i, _dp, _dp, _dp, sz]),
         "m1lya_batch_create": (i, [i, ip, ip, ip, C.POINTER(vp)]),
         "m1lya_batch_destroy": (None, [vp]),
+        "m1lya_batch_set_voigt_lut": (i, [vp, _dp, i, d, C.POINTER(LineConstants)]),
+        "m1lya_batch_prepare_subcycling": (i, [vp, i, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "m1lya_batch_last_error": (i, [vp, C.c_char_p, sz]),
         "m1lya_batch_count": (i, [vp]),
         "m1lya_batch_set_geometry": (i, [vp, i, C.POINTER(Geometry)]),

@@ -186,6 +186,24 @@ class M1LyaBatch:
     def set_graph(self, on: bool) -> None:
         self._check(self.lib.m1lya_batch_set_graph(self.handle, int(bool(on))))
 
+    # ---- device-side coefficient refresh ----
+    def set_voigt_lut(self, lut) -> None:
+        """Copies the Voigt tables of a VoigtLUT and the line constants to the device."""
+        from .m1lya import line_constants
+        t = np.ascontiguousarray(lut.t, dtype=np.float64)
+        self._check(self.lib.m1lya_batch_set_voigt_lut(self.handle, ptr(t), int(lut.nx), float(lut.xmax),
+                                                       C.byref(line_constants())))
+
+    def prepare_subcycling(self, idx: int, T, n_HI, alpha_c=None, alpha_c_F=None, js=None, pd=None,
+                           phi_all_bins: bool = True) -> None:
+        """PrepareLyαSubCycling! with Update_Hvoigt_And_ϕν_cell! on the device for one cloud."""
+        nn, nr, nz = self.dims[idx]
+        arrs = [None if v is None else np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (nz, nr)))
+                for v in (T, n_HI, alpha_c, alpha_c_F, js, pd)]
+        self._check(self.lib.m1lya_batch_prepare_subcycling(self.handle, int(idx),
+                                                            *[ptr(a) if a is not None else None for a in arrs],
+                                                            int(bool(phi_all_bins))))
+
     # ---- profiling ----
     def set_profiling(self, on: bool) -> None:
         self._check(self.lib.m1lya_batch_set_profiling(self.handle, int(on)))

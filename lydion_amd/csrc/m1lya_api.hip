@@ -58,6 +58,12 @@ struct m1lya_ctx {
     double *rcom = nullptr, *zcom = nullptr, *ivr = nullptr, *ivz = nullptr;
     bool geom_set = false;
 
+    // device-side coefficient refresh: Voigt tables, line constants, scratch inputs
+    double *lut = nullptr, *dnu_rel = nullptr, *dnuC_rel = nullptr, *nHI = nullptr, *acF = nullptr;
+    int lut_nx = 0;
+    double lut_xmax = 0.0;
+    LineConst lc{};
+
     // frequency grid
     double *nu = nullptr, *nuC = nullptr, *dnu = nullptr, *inv_dnu = nullptr, *inv_nuC = nullptr;
     int kcore_min = 0, kcore_max = 0, recoil = 0, flux = 0;
@@ -903,6 +909,64 @@ int m1lya_run_steps(m1lya_ctx* c, int n_steps, double c_tilde, double dt, double
                    "history");
         hipchk(c, hipStreamSynchronize(c->stream), "sync");
         check_err_flag(c, "run_steps");
+    });
+}
+
+int m1lya_set_voigt_lut(m1lya_ctx* c, const double* luts, int nx, double xmax, const m1lya_line_constants* lc)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (!luts || !lc || nx < 2 || !(xmax > 0.0)) fail(c, M1LYA_ERR_ARG, "set_voigt_lut: bad arguments");
+        if (c->lut && c->lut_nx != nx) fail(c, M1LYA_ERR_ARG, "set_voigt_lut: table size changed");
+        if (!c->lut) {
+            c->lut = dalloc(c, (size_t)5 * nx);
+            c->dnu_rel = dalloc(c, (size_t)c->nn + 1);
+            c->dnuC_rel = dalloc(c, (size_t)c->nn);
+            c->nHI = dalloc(c, c->ncell);
+            c->acF = dalloc(c, c->ncell);
+        }
+        hipchk(c, hipMemcpy(c->lut, luts, (size_t)5 * nx * sizeof(double), hipMemcpyHostToDevice), "lut");
+        // ν - ν_Lyα at the interfaces and centres, formed as the host setup forms them
+        std::vector<double> nu(c->nn + 1), nuC(c->nn), dr(c->nn + 1), dcr(c->nn);
+        hipchk(c, hipMemcpy(nu.data(), c->nu, nu.size() * sizeof(double), hipMemcpyDeviceToHost), "nu");
+        hipchk(c, hipMemcpy(nuC.data(), c->nuC, nuC.size() * sizeof(double), hipMemcpyDeviceToHost), "nuC");
+        for (int k = 0; k <= c->nn; ++k) dr[k] = nu[k] - lc->nu_lya;
+        for (int k = 0; k < c->nn; ++k) dcr[k] = nuC[k] - lc->nu_lya;
+        hipchk(c, hipMemcpy(c->dnu_rel, dr.data(), dr.size() * sizeof(double), hipMemcpyHostToDevice), "dnu_rel");
+        hipchk(c, hipMemcpy(c->dnuC_rel, dcr.data(), dcr.size() * sizeof(double), hipMemcpyHostToDevice), "dnuC_rel");
+        c->lut_nx = nx;
+        c->lut_xmax = xmax;
+        c->lc = LineConst{lc->b_prefact, lc->nu_over_c, lc->a_prefact, lc->sigma0_prefact};
+    });
+}
+
+int m1lya_prepare_subcycling(m1lya_ctx* c, const double* T, const double* n_HI, const double* alpha_c,
+                             const double* alpha_c_F, const double* js, const double* pd, int phi_all_bins)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (!c->lut) fail(c, M1LYA_ERR_STATE, "prepare_subcycling: Voigt tables not set (m1lya_set_voigt_lut)");
+        if (!T || !n_HI) fail(c, M1LYA_ERR_ARG, "prepare_subcycling: T and n_HI are required");
+        const size_t b = c->ncell * sizeof(double);
+        hipStream_t st = c->stream;
+        hipchk(c, hipMemcpyAsync(c->T, T, b, hipMemcpyHostToDevice, st), "T");
+        hipchk(c, hipMemcpyAsync(c->nHI, n_HI, b, hipMemcpyHostToDevice, st), "n_HI");
+        if (alpha_c) hipchk(c, hipMemcpyAsync(c->alpha_c, alpha_c, b, hipMemcpyHostToDevice, st), "alpha_c");
+        if (alpha_c_F) hipchk(c, hipMemcpyAsync(c->acF, alpha_c_F, b, hipMemcpyHostToDevice, st), "alpha_c_F");
+        if (js) hipchk(c, hipMemcpyAsync(c->js, js, b, hipMemcpyHostToDevice, st), "js");
+        if (pd) hipchk(c, hipMemcpyAsync(c->pd, pd, b, hipMemcpyHostToDevice, st), "pd");
+        RefreshArgs r;
+        r.nn = c->nn; r.ncell = (int)c->ncell; r.kmin = c->kcore_min; r.kmax = c->kcore_max;
+        r.phi_all_bins = phi_all_bins ? 1 : 0;
+        r.lut = c->lut; r.nx = c->lut_nx; r.xmax = c->lut_xmax; r.lc = c->lc;
+        r.dnu_rel = c->dnu_rel; r.dnuC_rel = c->dnuC_rel; r.dnu = c->dnu;
+        r.T = c->T; r.nHI = c->nHI; r.acF = alpha_c_F ? c->acF : nullptr;
+        r.Hv = c->Hv; r.phi = c->phi; r.aF = c->aF; r.alpha = c->alpha; r.dnuD = c->dnuD;
+        Prof p(c, "refresh");
+        launch_refresh(st, r);
+        p.done();
+        c->fij_stale = true;
+        hipchk(c, hipStreamSynchronize(st), "sync");
     });
 }
 

@@ -37,6 +37,7 @@
 
 #include "../../include/m1lya.h"
 #include "m1lya_fast.h"
+#include "m1lya_internal.h"
 
 #pragma clang fp contract(fast)
 
@@ -943,6 +944,13 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
     krange[2 * c + 1] = max(kmx_s, cl.kcore_max);
 }
 
+// FP64 -> FP32 copy (the batch stores the refreshed coefficient tables in FP32)
+__global__ void k_to_f32(float* __restrict__ dst, const double* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 __global__ void k_scale(float* a, float* b, float* c, size_t n, float s)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -982,6 +990,14 @@ struct m1lya_batch {
 
     size_t n3 = 0, nc = 0, nfr = 0, nfz = 0, nbin = 0, nivr = 0, nivz = 0;
     std::vector<CellRec> h_rec;            // host mirror of the cell records
+    std::vector<std::vector<double>> h_nu; // ν interfaces of each cloud (coefficient refresh)
+    // device-side coefficient refresh: Voigt tables, line constants and FP64 scratch
+    double* lut = nullptr;
+    int lut_nx = 0;
+    double lut_xmax = 0.0;
+    m1::LineConst lc{};
+    double* rscratch = nullptr;            // sized for the largest cloud
+    double nu_lya = 0.0;
     bool shear = false;                    // a cell's velocity gradient is sheared or anisotropic
     std::vector<BinRec> h_bin;
     CellRec* d_rec = nullptr;
@@ -1485,9 +1501,112 @@ int m1lya_batch_set_freq_grid(m1lya_batch* b, int cloud, const m1lya_freq_grid* 
         cl.recoil = g->recoil ? 1 : 0;
         cl.hll = g->flux_method == M1LYA_HLL ? 1 : 0;
         cl.alpha_floor = g->alpha_floor;
+        if ((int)b->h_nu.size() < b->n) b->h_nu.resize(b->n);
+        b->h_nu[cloud].assign(g->nu, g->nu + cl.nn + 1);
         b->grid_set[cloud] = 1;
         b->cl_dirty = true;
         b->krange_valid = false;
+    });
+}
+
+int m1lya_batch_set_voigt_lut(m1lya_batch* b, const double* luts, int nx, double xmax, const m1lya_line_constants* lc)
+{
+    return bguard(b, [&] {
+        if (!luts || !lc || nx < 2 || !(xmax > 0.0)) bfail(b, M1LYA_ERR_ARG, "set_voigt_lut: bad arguments");
+        if (b->lut && b->lut_nx != nx) bfail(b, M1LYA_ERR_ARG, "set_voigt_lut: table size changed");
+        if (!b->lut) {
+            size_t n3max = 0, ncmax = 0;
+            int nnmax = 0;
+            for (const Cloud& cl : b->cl) {
+                n3max = std::max(n3max, (size_t)cl.nn * cl.nr * cl.nz);
+                ncmax = std::max(ncmax, (size_t)cl.nr * cl.nz);
+                nnmax = std::max(nnmax, cl.nn);
+            }
+            b->lut = balloc<double>(b, (size_t)5 * nx);
+            // Hv, phi, aF [n3]; T, nHI, acF, alpha, dnuD [ncell]; dnu_rel, dnuC_rel, dnu [N_ν + 1]
+            b->rscratch = balloc<double>(b, 3 * n3max + 5 * ncmax + 3 * ((size_t)nnmax + 1));
+        }
+        bchk(b, hipMemcpy(b->lut, luts, (size_t)5 * nx * sizeof(double), hipMemcpyHostToDevice), "lut");
+        b->lut_nx = nx;
+        b->lut_xmax = xmax;
+        b->lc = m1::LineConst{lc->b_prefact, lc->nu_over_c, lc->a_prefact, lc->sigma0_prefact};
+        b->nu_lya = lc->nu_lya;
+    });
+}
+
+int m1lya_batch_prepare_subcycling(m1lya_batch* b, int cloud, const double* T, const double* n_HI,
+                                   const double* alpha_c, const double* alpha_c_F, const double* js,
+                                   const double* pd, int phi_all_bins)
+{
+    return bguard(b, [&] {
+        check_cloud(b, cloud);
+        if (!b->lut) bfail(b, M1LYA_ERR_STATE, "prepare_subcycling: Voigt tables not set (m1lya_batch_set_voigt_lut)");
+        if (!b->grid_set[cloud] || (int)b->h_nu.size() <= cloud || b->h_nu[cloud].empty())
+            bfail(b, M1LYA_ERR_STATE, "prepare_subcycling: frequency grid not set");
+        if (!T || !n_HI) bfail(b, M1LYA_ERR_ARG, "prepare_subcycling: T and n_HI are required");
+        const Cloud& cl = b->cl[cloud];
+        const int nn = cl.nn;
+        const size_t ncell = (size_t)cl.nr * cl.nz, n3 = ncell * nn;
+        size_t n3max = 0, ncmax = 0;
+        for (const Cloud& c2 : b->cl) {
+            n3max = std::max(n3max, (size_t)c2.nn * c2.nr * c2.nz);
+            ncmax = std::max(ncmax, (size_t)c2.nr * c2.nz);
+        }
+        double* Hv = b->rscratch;
+        double* phi = Hv + n3max;
+        double* aF = phi + n3max;
+        double* dT = aF + n3max;
+        double* dn = dT + ncmax;
+        double* dac = dn + ncmax;
+        double* dal = dac + ncmax;
+        double* ddD = dal + ncmax;
+        double* drel = ddD + ncmax;
+        double* dcrel = drel + (nn + 1);
+        double* ddnu = dcrel + (nn + 1);
+        // ν - ν_Lyα and Δν formed as the host setup forms them
+        const std::vector<double>& nu = b->h_nu[cloud];
+        std::vector<double> rel(nn + 1), crel(nn), dnu(nn);
+        for (int k = 0; k <= nn; ++k) rel[k] = nu[k] - b->nu_lya;
+        for (int k = 0; k < nn; ++k) {
+            dnu[k] = nu[k + 1] - nu[k];
+            crel[k] = (nu[k] + 0.5 * dnu[k]) - b->nu_lya;
+        }
+        hipStream_t st = b->stream;
+        bchk(b, hipMemcpyAsync(drel, rel.data(), rel.size() * sizeof(double), hipMemcpyHostToDevice, st), "dnu_rel");
+        bchk(b, hipMemcpyAsync(dcrel, crel.data(), crel.size() * sizeof(double), hipMemcpyHostToDevice, st), "dnuC_rel");
+        bchk(b, hipMemcpyAsync(ddnu, dnu.data(), dnu.size() * sizeof(double), hipMemcpyHostToDevice, st), "dnu");
+        bchk(b, hipMemcpyAsync(dT, T, ncell * sizeof(double), hipMemcpyHostToDevice, st), "T");
+        bchk(b, hipMemcpyAsync(dn, n_HI, ncell * sizeof(double), hipMemcpyHostToDevice, st), "n_HI");
+        if (alpha_c_F)
+            bchk(b, hipMemcpyAsync(dac, alpha_c_F, ncell * sizeof(double), hipMemcpyHostToDevice, st), "alpha_c_F");
+        m1::RefreshArgs r;
+        r.nn = nn; r.ncell = (int)ncell; r.kmin = cl.kcore_min; r.kmax = cl.kcore_max;
+        r.phi_all_bins = phi_all_bins ? 1 : 0;
+        r.lut = b->lut; r.nx = b->lut_nx; r.xmax = b->lut_xmax; r.lc = b->lc;
+        r.dnu_rel = drel; r.dnuC_rel = dcrel; r.dnu = ddnu;
+        r.T = dT; r.nHI = dn; r.acF = alpha_c_F ? dac : nullptr;
+        r.Hv = Hv; r.phi = phi; r.aF = aF; r.alpha = dal; r.dnuD = ddD;
+        m1::launch_refresh(st, r);
+        const Arrays& A = b->A;
+        const unsigned nb = (unsigned)std::min<size_t>((n3 + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, st, A.Hv + cl.o3, Hv, n3);
+        hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, st, A.phi + cl.o3, phi, n3);
+        hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, st, A.aF + cl.o3, aF, n3);
+        bchk(b, hipGetLastError(), "refresh");
+        std::vector<double> al(ncell), dD(ncell);
+        bchk(b, hipMemcpyAsync(al.data(), dal, ncell * sizeof(double), hipMemcpyDeviceToHost, st), "alpha");
+        bchk(b, hipMemcpyAsync(dD.data(), ddD, ncell * sizeof(double), hipMemcpyDeviceToHost, st), "dnuD");
+        bchk(b, hipStreamSynchronize(st), "hipStreamSynchronize");
+        CellRec* rec = b->h_rec.data() + cl.oc;
+        for (size_t c = 0; c < ncell; ++c) {
+            rec[c].alpha = al[c];
+            rec[c].dnuD = dD[c];
+            rec[c].T = T[c];
+            if (alpha_c) rec[c].alpha_c = alpha_c[c];
+            if (js) rec[c].js = js[c];
+            if (pd) rec[c].pd = pd[c];
+        }
+        push_records(b, cloud);
     });
 }
 

@@ -224,3 +224,36 @@ void launch_nonfinite(hipStream_t s, const double* x, size_t n, int* flag);
 void launch_selftest(hipStream_t s, int op, const double* a, const double* b, double* out, size_t n);
 
 }  // namespace m1
+
+namespace m1 {
+
+// ---- device-side coefficient refresh (k_refresh in m1lya_kernels.hip) ----
+struct LineConst {
+    double b_prefact;       // b(T) = b_prefact √T
+    double nu_over_c;       // ΔνD = b ν_Lyα / c
+    double a_prefact;       // a = a_prefact / b
+    double sigma0_prefact;  // σ0 = sigma0_prefact / b
+};
+
+struct RefreshArgs {
+    int nn, ncell, kmin, kmax, phi_all_bins;
+    const double* lut;      // [5][nx]: erf, dawson, exp(-x²), x exp(-x²), (1-2x²) exp(-x²)
+    int nx;
+    double xmax;
+    LineConst lc;
+    const double* dnu_rel;  // ν - ν_Lyα at the N_ν + 1 interfaces
+    const double* dnuC_rel; // νC - ν_Lyα
+    const double* dnu;
+    const double* T;        // [ncell] input temperature
+    const double* nHI;      // [ncell] input H I density
+    const double* acF;      // [ncell] continuum flux opacity added to every bin of α_F (may be null)
+    double* Hv;             // outputs
+    double* phi;
+    double* aF;
+    double* alpha;
+    double* dnuD;
+};
+
+void launch_refresh(hipStream_t s, const RefreshArgs& a);
+
+}  // namespace m1

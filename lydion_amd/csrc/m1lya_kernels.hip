@@ -830,3 +830,108 @@ void launch_selftest(hipStream_t s, int op, const double* a, const double* b, do
 }
 
 }  // namespace m1
+
+namespace m1 {
+
+// ---------------------------------------------------------------------------
+// Device-side coefficient refresh: the temperature- and density-dependent part of
+// PrepareLyαSubCycling! (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2090-2155)
+// with Update_Hvoigt_And_ϕν_cell! (:2039-2082).  Per cell: a(T), ΔνD(T), the bin-averaged
+// Voigt profile Htophat (src/ConstantsAndFunctions.jl:227-279) from the lookup tables of
+// CreateVoigtLUTs with the evaluators of src/FastInterpolations.jl:83-274, ϕ_ν normalised
+// over all bins (the RHD form) or over the core bins (Update_ϕν!,
+// src/ConstantsAndFunctions.jl:362-393), α = n_HI σ0(T), α_F = α Hvoigt + α_c,F.
+// The formulas are evaluated in the order the host setup evaluates them, so the
+// tables agree with the host's bit for bit.  One wavefront per cell, lanes over bins;
+// the normalisation sum runs on lane 0 in bin order.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double lut_eval(const double* y, int nx, double xmin, double inv_dx, double x)
+{
+    const double xt = (x - xmin) * inv_dx;
+    long i = (long)floor(xt);
+    if (i + 2 > nx) i = nx - 2;
+    const double d = (double)(i + 1) - xt;
+    return d * y[i] + (1.0 - d) * y[i + 1];
+}
+
+struct VLut {
+    const double* t;
+    int nx;
+    double xmax, xmin, inv_dx;
+    __device__ double fn(int which, double x) const
+    {
+        if (fabs(x) >= xmax) {
+            if (which == 0) return copysign(1.0, x);
+            if (which == 1) {
+                const double ix = 1.0 / x;
+                return 0.5 * ix + 0.25 * (ix * ix * ix);
+            }
+            return 0.0;
+        }
+        return lut_eval(t + (size_t)which * nx, nx, xmin, inv_dx, x);
+    }
+};
+
+// H(x,a) and its integral, Smith et al. 2018 eqs. 12-13 (src/ConstantsAndFunctions.jl:227-256)
+__device__ double voigt_H(const VLut& V, double x, double a)
+{
+    const double sp2 = sqrt(kPi) / 2.0;
+    const double isp2 = 1.0 / sp2;
+    return V.fn(2, x) + isp2 * a * (2.0 * x * V.fn(1, x) - 1.0) + (a * a) * V.fn(4, x);
+}
+__device__ double voigt_intH(const VLut& V, double x, double a)
+{
+    const double sp2 = sqrt(kPi) / 2.0;
+    const double isp2 = 1.0 / sp2;
+    return sp2 * V.fn(0, x) - isp2 * a * V.fn(1, x) + (a * a) * V.fn(3, x);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_refresh(RefreshArgs a)
+{
+    extern __shared__ double sHv[];
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int nn = a.nn;
+    const size_t base = (size_t)c * nn;
+    VLut V;
+    V.t = a.lut; V.nx = a.nx; V.xmax = a.xmax; V.xmin = -a.xmax;
+    V.inv_dx = 1.0 / ((a.xmax - (-a.xmax)) / (double)(a.nx - 1));
+    const double T = a.T[c];
+    const double b = a.lc.b_prefact * sqrt(T);
+    const double dnuD = b * a.lc.nu_over_c;
+    const double ic = 1.0 / dnuD;
+    const double av = a.lc.a_prefact / b;
+    const double alpha = a.nHI[c] * (a.lc.sigma0_prefact / b);
+    const double acF = a.acF ? a.acF[c] : 0.0;
+    for (int k = lane; k < nn; k += 64) {
+        const double x = a.dnuC_rel[k] * ic, xmx = a.dnu_rel[k + 1] * ic, xmn = a.dnu_rel[k] * ic;
+        const double dx = xmx - xmn;
+        const double h = (dx > 1e-5) ? (voigt_intH(V, xmx, av) - voigt_intH(V, xmn, av)) / dx : voigt_H(V, x, av);
+        sHv[k] = h;
+        a.Hv[base + k] = h;
+        a.aF[base + k] = alpha * h + acF;
+    }
+    __syncthreads();
+    const int k0 = a.phi_all_bins ? 0 : a.kmin, k1 = a.phi_all_bins ? nn - 1 : a.kmax;
+    if (lane == 0) {
+        double s = 0.0;
+        for (int k = k0; k <= k1; ++k) s += a.dnu[k] * sHv[k];
+        sHv[nn] = 1.0 / s;
+        a.alpha[c] = alpha;
+        a.dnuD[c] = dnuD;
+    }
+    __syncthreads();
+    const double inv = sHv[nn];
+    for (int k = lane; k < nn; k += 64) a.phi[base + k] = (k >= k0 && k <= k1) ? sHv[k] * inv : 0.0;
+}
+
+void launch_refresh(hipStream_t s, const RefreshArgs& a)
+{
+    hipLaunchKernelGGL(k_refresh, dim3(a.ncell), dim3(64), (size_t)(a.nn + 1) * sizeof(double), s, a);
+}
+
+}  // namespace m1

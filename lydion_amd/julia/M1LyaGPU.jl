@@ -18,7 +18,8 @@ module M1LyaGPU
 
 export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
-       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!
+       Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!,
+       set_voigt_lut!, prepare_subcycling!
 
 const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
 
@@ -297,6 +298,34 @@ function adjoint_run!(Lyα, nsteps::Integer, c̃::Float64, Δt::Float64, Z_cente
         mf === C_NULL ? Ptr{Float64}(C_NULL) : pointer(mf), sp === C_NULL ? Ptr{Float64}(C_NULL) : pointer(sp),
         checkpoint_every, obj, hist, gs, gc, gb), ctx)
     return obj[], hist, gs, gc, gb
+end
+
+# --- device-side coefficient refresh (m1lya_set_voigt_lut, m1lya_prepare_subcycling) ---
+
+struct CLineConstants      # m1lya_line_constants
+    nu_lya::Float64; b_prefact::Float64; nu_over_c::Float64; a_prefact::Float64; sigma0_prefact::Float64
+end
+
+"""Copies the Voigt tables of CreateVoigtLUTs (src/FastInterpolations.jl:26-80) to the device:
+`luts` is [nx, 5] (erf, dawson, exp(-x²), x exp(-x²), (1-2x²) exp(-x²)) on [-xmax, xmax]."""
+function set_voigt_lut!(Lyα, luts::Matrix{Float64}, xmax::Float64, lc::CLineConstants)
+    ctx = ctxof(Lyα)
+    GC.@preserve luts check(ccall((:m1lya_set_voigt_lut, libm1lya), Cint,
+                                  (Ptr{Cvoid}, Ptr{Float64}, Cint, Float64, Ref{CLineConstants}),
+                                  ctx, luts, size(luts, 1), xmax, lc), ctx)
+    return nothing
+end
+
+# --- src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2090-2155, for all cells at once ---
+"""PrepareLyαSubCycling! on the device: Hvoigt, ϕ_ν (over all bins), ΔνD, α_Lyα and α_F from
+T and n_HI; αc (absorption), αc_F (flux), js and pd come from the thermochemistry."""
+function prepare_subcycling!(Lyα, T::Matrix{Float64}, nHI::Matrix{Float64}; αc=nothing, αc_F=nothing, js=nothing,
+                             pd=nothing, all_bins::Bool=true)
+    ctx = ctxof(Lyα)
+    GC.@preserve T nHI αc αc_F js pd check(ccall((:m1lya_prepare_subcycling, libm1lya), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
+        ctx, T, nHI, optr(αc), optr(αc_F), optr(js), optr(pd), Cint(all_bins)), ctx)
+    return nothing
 end
 
 end # module

@@ -17,6 +17,12 @@ from ._lib import FIELD, OUTFLOW, REFLECT, check, ptr
 from .grid import GeometryCylindrical
 from .lya_arrays import LyaArrays
 
+
+def line_constants() -> "_lib.LineConstants":
+    """m1lya_line_constants from the physical constants of the host setup."""
+    from . import constants as K
+    return _lib.LineConstants(K.nu_Lya, K.b_PREFACT, K.nu_Lya_over_c, K.a_PREFACT, K.sigma0_PREFACT)
+
 # LyaArrays key -> C field id
 _KEYS = {
     "J": "J", "J_old": "J_OLD", "H_r": "H_R", "H_z": "H_Z", "H_r_old": "H_R_OLD", "H_z_old": "H_Z_OLD",
@@ -125,6 +131,22 @@ class M1LyaContext:
 
     def reset_kernel_stats(self) -> None:
         self._check(self.lib.m1lya_reset_kernel_stats(self.handle))
+
+    # ---- device-side coefficient refresh ----
+    def set_voigt_lut(self, lut) -> None:
+        """Copies the Voigt tables of a VoigtLUT (CreateVoigtLUTs) and the line constants."""
+        t = np.ascontiguousarray(lut.t, dtype=np.float64)
+        self._check(self.lib.m1lya_set_voigt_lut(self.handle, ptr(t), int(lut.nx), float(lut.xmax),
+                                                 C.byref(line_constants())))
+
+    def prepare_subcycling(self, T, n_HI, alpha_c=None, alpha_c_F=None, js=None, pd=None,
+                           phi_all_bins: bool = True) -> None:
+        """PrepareLyαSubCycling! with Update_Hvoigt_And_ϕν_cell! on the device for every cell."""
+        cell = (self.n_z, self.n_r)
+        arrs = [None if v is None else np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), cell))
+                for v in (T, n_HI, alpha_c, alpha_c_F, js, pd)]
+        self._check(self.lib.m1lya_prepare_subcycling(self.handle, *[ptr(a) if a is not None else None for a in arrs],
+                                                      int(bool(phi_all_bins))))
 
 
 # ---------------------------------------------------------------------------
