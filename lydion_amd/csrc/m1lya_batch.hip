@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         // trace times the rate for every bin, so VEL is the same in all bins and the shared-G
         // interface form below equals the reference rows.  Otherwise VEL differs between bins and
         // each row is built as the reference builds it, with its own bin's VEL in the exponents of
-        // its three neighbours (src/FreqFactors.jl:142-278).
+        // its three neighbours (src/FreqFactors.jl:226-254).
         const bool aniso =
             kShear && vel && !((cr.durdz + cr.duzdr) == 0.0 && cr.durdr == cr.duzdz && cr.duzdz == cr.urr);
         const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
