@@ -4,7 +4,11 @@
 
 hipcc cross-compiles for gfx950 without a GPU.  FP contraction is disabled so
 that the FP64 step rounds exactly as its formulas are written, and packed FP32
-instructions are disabled (-packed-fp32-ops).
+instructions are disabled (-packed-fp32-ops).  Machine-level loop-invariant code
+motion is disabled: in the long cell-marching loops it hoists every constant of
+the inlined exp / division sequences into a VGPR held across the loop (about 30
+VGPRs in the batched step), which costs occupancy or spills; rematerialising the
+constants inside the loop is scalar work that is cheaper.
 """
 from __future__ import annotations
 
@@ -28,6 +32,7 @@ COMMON_FLAGS = [
     "-std=c++17",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-disable-machine-licm",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -39,7 +39,11 @@
 #include "m1lya_fast.h"
 #include "m1lya_internal.h"
 
-#pragma clang fp contract(fast)
+// FP contraction is on only inside the FP32 advection (m1lya_fast.h and the cell update
+// below).  The FP64 emission, frequency factors, solve and reductions round as written:
+// with contraction the backend fuses a multiply into an add or not depending on the code
+// around it, so kernel variants that share this code (the velocity-shear variant, for
+// one) would round it differently and a cloud's result would depend on its batch.
 
 namespace mb {
 
@@ -152,24 +156,6 @@ __device__ __forceinline__ double wave_sum64(double v)
 // 1/x from the hardware estimate (relative error ~2^-24): drcp0 as is, for values that
 // only set a relaxation rate; drcp1 after one Newton step (~2e-15), for pivots and
 // quantities that set an equilibrium; drcp after two (a few ulp).
-// OneMinusϕ_L_and_χ_L / OneMinusϕ_R_and_χ_R (src/FreqFactors.jl:54-133) in FP64
-__device__ __forceinline__ void omphi_L64(double P, double& o, double& chi)
-{
-    if (P >= kPmax) { o = exp(-kPmax); chi = (1.0 - o) / kPmax; return; }
-    if (P <= -kPmax) { o = exp(kPmax); chi = (1.0 - o) / (-kPmax); return; }
-    if (fabs(P) < kPmin) { o = 1.0 - P + 0.5 * (P * P); chi = 1.0 - 0.5 * P; return; }
-    o = exp(-P);
-    chi = (1.0 - o) / P;
-}
-__device__ __forceinline__ void omphi_R64(double P, double& o, double& chi)
-{
-    if (P >= kPmax) { o = exp(kPmax); chi = (1.0 - o) / kPmax; return; }
-    if (P <= -kPmax) { o = exp(-kPmax); chi = (1.0 - o) / (-kPmax); return; }
-    if (fabs(P) < kPmin) { o = 1.0 + P + 0.5 * (P * P); chi = -1.0 - 0.5 * P; return; }
-    o = exp(P);
-    chi = (1.0 - o) / P;
-}
-
 __device__ __forceinline__ double drcp0(double x) { return __builtin_amdgcn_rcp(x); }
 
 __device__ __forceinline__ double drcp1(double x)
@@ -185,6 +171,24 @@ __device__ __forceinline__ double drcp(double x)
     r = fma(r, e, r);
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
+}
+
+// OneMinusϕ_L_and_χ_L / OneMinusϕ_R_and_χ_R (src/FreqFactors.jl:54-133) in FP64
+__device__ __forceinline__ void omphi_L64(double P, double& o, double& chi)
+{
+    if (P >= kPmax) { o = exp(-kPmax); chi = (1.0 - o) * (1.0 / kPmax); return; }
+    if (P <= -kPmax) { o = exp(kPmax); chi = (1.0 - o) * (-1.0 / kPmax); return; }
+    if (fabs(P) < kPmin) { o = 1.0 - P + 0.5 * (P * P); chi = 1.0 - 0.5 * P; return; }
+    o = exp(-P);
+    chi = (1.0 - o) * drcp1(P);
+}
+__device__ __forceinline__ void omphi_R64(double P, double& o, double& chi)
+{
+    if (P >= kPmax) { o = exp(kPmax); chi = (1.0 - o) * (1.0 / kPmax); return; }
+    if (P <= -kPmax) { o = exp(-kPmax); chi = (1.0 - o) * (-1.0 / kPmax); return; }
+    if (fabs(P) < kPmin) { o = 1.0 + P + 0.5 * (P * P); chi = -1.0 - 0.5 * P; return; }
+    o = exp(P);
+    chi = (1.0 - o) * drcp1(P);
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
@@ -283,6 +287,9 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
                                        double duzdz, double urr, double PREF, double R0, int recoil, double dw,
                                        double& ra, double& rb, double& rc, double& rd)
 {
+    // contracted: only the shear variant of k_step holds this code, so every cell that takes
+    // it rounds it the same way, and the fused form needs fewer registers
+#pragma clang fp contract(fast)
     double Dr, Dz, Dx;
     D_edd64(hr, hz, jo, Dr, Dz, Dx);
     double Dpp = 1.0 - Dr - Dz;
@@ -290,7 +297,7 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
     const double v = (Dr * durdr + Dx * (durdz + duzdr) + Dz * duzdz + Dpp * urr) * kInvC;
     auto expo = [&](int x, double half, double& qx) {
         const double Kx = PREF * (double)Hv[x];
-        qx = half * (1.0 / Kx);
+        qx = half * drcp1(Kx);
         double Cx = v * bins[x].nuC;
         if (recoil) Cx += Kx * (R0 - 2.0 * bins[x].inv_nuC);
         return fmin(fmax(Cx * qx, -kPmax), kPmax);
@@ -298,14 +305,20 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
     double qk, oLk, xLk, oRk, xRk;
     const double Pk = expo(k, bins[k].half_R, qk);
     omphi_L64(Pk, oLk, xLk);
-    omphi_R64(Pk, oRk, xRk);
+    if (fabs(Pk) < kPmin || fabs(Pk) >= kPmax) {
+        omphi_R64(Pk, oRk, xRk);
+    } else {
+        // exp(P) = 1 / exp(-P): one exponential for both sides of bin k
+        oRk = drcp1(oLk);
+        xRk = (1.0 - oRk) * drcp1(Pk);
+    }
     const double idn = bins[k].inv_dnu;
     double qkp, oRkp, xRkp;
     omphi_R64(expo(k + 1, bins[k + 1].half_L, qkp), oRkp, xRkp);
-    const double den_kp = idn * (1.0 / (xLk * qk - xRkp * qkp));
+    const double den_kp = idn * drcp1(xLk * qk - xRkp * qkp);
     double qkm, oLkm, xLkm;
     omphi_L64(expo(k - 1, bins[k - 1].half_R, qkm), oLkm, xLkm);
-    const double den_km = idn * (1.0 / (xLkm * qkm - xRk * qk));
+    const double den_km = idn * drcp1(xLkm * qkm - xRk * qk);
     ra = (k == 1) ? 0.0 : oLkm * den_km;
     rc = (k == nn - 2) ? 0.0 : oRkp * den_kp;
     rb = -((oLk * den_kp + oRk * den_km) + dw);
@@ -313,7 +326,7 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
 }
 
 template <bool kShear>
-__global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+__global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ clouds, Arrays A,
                                                  const Seg* __restrict__ segs, int nblk, int wpb, int parity,
                                                  const int* __restrict__ krange, double* __restrict__ part)
 {
@@ -513,6 +526,7 @@ __global__ __launch_bounds__(256, 3) void k_step(const Cloud* __restrict__ cloud
         // ---- cell update (FP32): src/AdvectJandH.jl:1319-1379 ----
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
+#pragma clang fp contract(fast)
             jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
             if (!L.val[q]) continue;
             const int k = L.kb[q];
@@ -1848,8 +1862,9 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         b->profiling = false;
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         set_ctrl(b, nullptr, 0);
-        // segment lengths: powers of two and the lengths that give 3 or 6 wavefronts per
-        // SIMD on 256-cell rows; 2 or 4 wavefronts per block
+        // segment lengths: powers of two (8 gives 4 wavefronts per SIMD, the kernel's
+        // occupancy, on 128 rows of 256 cells) and the lengths that give 3 or 6 wavefronts
+        // per SIMD on 256-cell rows; 2 or 4 wavefronts per block
         static const int seg_opts[] = {6, 8, 11, 16, 22, 32, 43, 64};
         static const int wpb_opts[] = {2, 4};
         int best_s = b->seg_len, best_w = b->wpb;

@@ -279,3 +279,7 @@ __device__ __forceinline__ void causal_limit(float J, float& Hr, float& Hz)
 }
 
 }  // namespace m1f
+
+// contraction ends with this header: code that includes it rounds as written again (the
+// library is built with -ffp-contract=off; this target has no float_control push / pop)
+#pragma clang fp contract(off)

@@ -128,6 +128,24 @@ def test_batch_equals_single_clouds(lut):
             assert bitdiff(big.download(idx, k), one.download(0, k)) == 0, (idx, k)
 
 
+def test_batch_with_sheared_cloud_equals_single_clouds(lut):
+    """A batch holding a sheared-flow cloud runs the kernel variant that can build reference
+    frequency rows per bin; every other cloud of that batch (static, cone flow, isotropic
+    expansion with 128 bins) still gets bit for bit the result it gets alone."""
+    from lydion_amd import workloads as W
+    clouds = _mixed_clouds(lut) + [developed_state(W.rhd_like_cloud(24, 40, 32, lut=lut))]
+    big = M1LyaBatch(clouds)
+    hb = big.run_steps(6)
+    for idx, cl in enumerate(clouds[:-1]):
+        one = M1LyaBatch([cl])
+        h1 = one.run_steps(6)
+        assert np.array_equal(hb[:, idx, :], h1[:, 0, :]), idx
+        for k in ("J", "H_r", "H_z", "f_r", "f_z"):
+            assert bitdiff(big.download(idx, k), one.download(0, k)) == 0, (idx, k)
+        one.close()
+    big.close()
+
+
 def test_mixed_batch_vs_fp64(lut):
     """Every cloud of an uneven batch (bins 32..128, τ0 1e5..1e9, outflow, dust) against its FP64 run."""
     clouds = _mixed_clouds(lut)
