@@ -1021,7 +1021,7 @@ struct m1lya_batch {
     bool krange_valid = false;
 
     // tile shape and the work / partial-row tables it implies
-    int seg_len = 22, wpb = 4;
+    int seg_len = 16, wpb = 4;   // the autotuner's choice on 256x256 and 64-cloud batches
     std::vector<Seg> segs;
     Seg* d_segs = nullptr;
     size_t segs_cap = 0;

@@ -36,9 +36,11 @@ constexpr int kIvStride = 6;
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ float fmaxf_(float a, float b) { return (b > a) ? b : a; }
-__device__ __forceinline__ float fminf_(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float fclamp(float x, float lo, float hi) { return (x > hi) ? hi : ((x < lo) ? lo : x); }
+// max / min / clamp as one v_med3_f32 each (the median of (a, b, +inf) is max(a, b));
+// the state holds no NaN, and med3 needs no canonicalising of its inputs
+__device__ __forceinline__ float fmaxf_(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, __builtin_inff()); }
+__device__ __forceinline__ float fminf_(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -__builtin_inff()); }
+__device__ __forceinline__ float fclamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
 
 // MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign
 // (sign bits compared, so no product can underflow; a zero argument gives 0).
