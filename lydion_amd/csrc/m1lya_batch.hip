@@ -335,6 +335,9 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
     // Z interfaces j0 .. j0+wpb (F_J, F_Hr, F_HZ), bin k of lane l at [q*64 + l].
     __shared__ float zrow[kMaxWaves + 4][3][kMaxBins];
     __shared__ float zflx[kMaxWaves + 1][3][kMaxBins];
+    // the cloud's frequency-bin constants, read every cell: from LDS rather than through the
+    // cache hierarchy, whose latency the column work would otherwise wait on
+    __shared__ BinRec sbin[kMaxBins];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63;
@@ -376,7 +379,9 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
     const float* ucr = A.uc_r + cl.ofr;
     const float* ucz = A.uc_z + cl.ofz;
     const CellRec* rec = A.rec + cl.oc;
-    const BinRec* bins = A.bin + cl.obin;
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) sbin[t] = A.bin[cl.obin + t];
+    __syncthreads();
+    const BinRec* bins = sbin;
     const int kmin = krange[2 * cidx], kmax = krange[2 * cidx + 1];
 
     LaneBins L;
@@ -438,6 +443,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
         const size_t cell = (size_t)j * nr + i;   // used only for valid rows, i >= i0
         const size_t base = cell * nn;
 
+        float af_c[2] = {0.f, 0.f};   // α_F of cell i (the left side of its right r interface)
         // ---- right r interface of cell i (FP32) ----
         float FR[2][3], kip[2], kimN[2];
 #pragma unroll
@@ -454,12 +460,13 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
                 ld_r(J, Hr, Hz, nn, nr, j, k, i + 2, bcr, wJ[q][3], vr, vz);
                 m1f::flux_ratios(wJ[q][3], vr, vz, wR[q][3], wZ[q][3]);
             }
+            af_c[q] = aF[((size_t)j * nr + max(i, 0)) * nn + k];
             if (!L.act[q]) continue;
             m1f::Stencil32 st;
             st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
             st.fr[0] = wR[q][1]; st.fr[1] = wR[q][2]; st.fr[2] = wR[q][3]; st.fr[3] = wR[q][0];
             st.fz[0] = wZ[q][1]; st.fz[1] = wZ[q][2]; st.fz[2] = wZ[q][3]; st.fz[3] = wZ[q][0];
-            const float aL = aF[((size_t)j * nr + max(i, 0)) * nn + k];
+            const float aL = af_c[q];
             const float aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
             const float u = ucr[(size_t)j * (nr + 1) + i + 1];
             const m1f::Face32 f = m1f::interface_flux(0, hll, st, ivr + (size_t)(i + 1) * m1f::kIvStride, aL, aR, u);
@@ -575,7 +582,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
             hv[q] = h;
             const double aJ = cr.alpha_c + al_pd * h;
             const double jn = (jo[q] + src * (double)php[idx]) * drcp1(1.0 + aJ * cdt);
-            const double xH = (double)aF[idx] * cdt;
+            const double xH = (double)af_c[q] * cdt;
             const double gH = xH * drcp1(1.0 + xH);          // 1 - 1/(1+x)
             const double dk = bins[L.kb[q]].dnu;
             sr += dk * (hr[q] * gH);
