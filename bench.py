@@ -57,7 +57,7 @@ WORKLOAD = ("cone cloud 256x256 r-Z, 128 bins, tau0=1e8, T=1e4 K, cone half-angl
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--steps", type=int, default=None, help="timed steps (default 100; rhd: 200 substeps)")
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     p.add_argument("--workload", choices=("cone", "sweep", "adjoint", "rhd"), default="cone",
@@ -361,6 +361,8 @@ def main_rhd(args):
 
     cloud = developed_state(W.rhd_like_cloud())
     n = args.steps
+    if args.copy_calibration:
+        copy_calibration()
     upd = cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
     wall, stats, graph_wall, tiles = run_fp32([cloud], n, args.warmup, not args.no_autotune)
     value = upd * n / wall
@@ -402,6 +404,8 @@ def main_rhd(args):
 
 def main():
     args = parse()
+    if args.steps is None:
+        args.steps = 200 if args.workload == "rhd" else 100
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding of a cloud is a design note only)", file=sys.stderr)

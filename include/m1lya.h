@@ -321,7 +321,7 @@ int  m1lya_batch_run_steps(m1lya_batch *b, int n_steps, double *history);
 int  m1lya_batch_scalars(m1lya_batch *b, double *out);
 /* [n_clouds][2] active-bin ranges the next step uses (0-based, inclusive). */
 int  m1lya_batch_active_bins(m1lya_batch *b, int *out);
-/* Tile shape: cells per wavefront segment along r, wavefronts per block (1..4).
+/* Tile shape: cells per wavefront segment along r, wavefronts per block (1..8).
  * The state after a step does not depend on the tile shape. */
 int  m1lya_batch_set_tiles(m1lya_batch *b, int seg_len, int waves_per_block);
 int  m1lya_batch_get_tiles(const m1lya_batch *b, int *seg_len, int *waves_per_block);

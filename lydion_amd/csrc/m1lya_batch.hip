@@ -58,7 +58,7 @@ constexpr double kPmin = 1e-3, kPmax = 40.0;
 constexpr int kMaxBins = 128;
 constexpr int kNScal = 5;                        // MF, L, net, out, esc
 constexpr int kPartStride = 2 * kMaxBins + 8;    // e1[N_ν], e10[N_ν], scalars
-constexpr int kMaxWaves = 4;
+constexpr int kMaxWaves = 8;
 constexpr int kSlices = 16;                      // row slices of the first reduction stage
 
 // Per-cell inputs of the step, one 160-byte record per cell (read with scalar loads).
@@ -325,19 +325,32 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
     rd = -dw * jo;
 }
 
-template <bool kShear>
-__global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ clouds, Arrays A,
-                                                 const Seg* __restrict__ segs, int nblk, int wpb, int parity,
-                                                 const int* __restrict__ krange, double* __restrict__ part)
+// LDS of a k_step block of wpb wavefronts (dynamic, so that 4-wavefront blocks keep four
+// blocks per CU while 8-wavefront blocks take two):
+//   sbin  BinRec[kMaxBins]              the cloud's frequency-bin constants, read every cell
+//   sh    double[wpb][kPartStride]      per-wavefront active-bin energies and step sums
+//   sacc  double[wpb][2 kNScal]         per-wavefront compensated sums (s, c)
+//   zrow  float[wpb + 4][3][kMaxBins]   Z stencil rows j0-2 .. j0+wpb+1: J, H_r/J, H_Z/J
+//   zflx  float[wpb + 1][3][kMaxBins]   Z interfaces j0 .. j0+wpb: F_J, F_Hr, F_HZ
+// (bin k of lane l at [q*64 + l] in zrow and zflx).
+constexpr size_t step_lds_bytes(int wpb)
 {
-    __shared__ double sh[kMaxWaves][kPartStride];
-    // Z stencil exchange of the block: rows j0-2 .. j0+wpb+1 (J, H_r/J, H_Z/J) and the
-    // Z interfaces j0 .. j0+wpb (F_J, F_Hr, F_HZ), bin k of lane l at [q*64 + l].
-    __shared__ float zrow[kMaxWaves + 4][3][kMaxBins];
-    __shared__ float zflx[kMaxWaves + 1][3][kMaxBins];
-    // the cloud's frequency-bin constants, read every cell: from LDS rather than through the
-    // cache hierarchy, whose latency the column work would otherwise wait on
-    __shared__ BinRec sbin[kMaxBins];
+    return sizeof(BinRec) * kMaxBins + sizeof(double) * (size_t)wpb * (kPartStride + 2 * kNScal) +
+           sizeof(float) * 3 * kMaxBins * (size_t)(2 * wpb + 5);
+}
+
+template <bool kShear>
+__global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+                                                            const Seg* __restrict__ segs, int nblk, int wpb,
+                                                            int parity, const int* __restrict__ krange,
+                                                            double* __restrict__ part)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    BinRec* sbin = reinterpret_cast<BinRec*>(smem);
+    double (*sh)[kPartStride] = reinterpret_cast<double (*)[kPartStride]>(sbin + kMaxBins);
+    double (*sacc)[2 * kNScal] = reinterpret_cast<double (*)[2 * kNScal]>(&sh[wpb][0]);
+    float (*zrow)[3][kMaxBins] = reinterpret_cast<float (*)[3][kMaxBins]>(&sacc[wpb][0]);
+    float (*zflx)[3][kMaxBins] = zrow + (wpb + 4);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63;
@@ -348,7 +361,6 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
 
     // per-wavefront compensated sums of M_F, L, radial net/out and f_esc, kept by lane 0
     // in LDS (s, c pairs) rather than in registers of every lane
-    __shared__ double sacc[kMaxWaves][2 * kNScal];
     enum { S_MF = 0, S_L, S_NET, S_OUT, S_ESC };
     if (lane < 2 * kNScal) sacc[wave][lane] = 0.0;
     auto kadd = [&](int m, double x) {
@@ -824,7 +836,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const Cloud* __restrict__ cloud
 
 // Active-bin energies straight from the state (before the first step after an
 // upload): block rows in k_step's format, scalars zero.
-__global__ __launch_bounds__(256) void k_active_rows(const Cloud* __restrict__ clouds, Arrays A,
+__global__ __launch_bounds__(64 * kMaxWaves) void k_active_rows(const Cloud* __restrict__ clouds, Arrays A,
                                                      const Seg* __restrict__ segs, int nblk, int wpb, int parity,
                                                      double* __restrict__ part)
 {
@@ -1028,7 +1040,7 @@ struct m1lya_batch {
     bool krange_valid = false;
 
     // tile shape and the work / partial-row tables it implies
-    int seg_len = 16, wpb = 4;   // the autotuner's choice on 256x256 and 64-cloud batches
+    int seg_len = 16, wpb = 8;   // the autotuner's choice on the 256x256x128 cone cloud
     std::vector<Seg> segs;
     Seg* d_segs = nullptr;
     size_t segs_cap = 0;
@@ -1237,11 +1249,12 @@ void launch_step_kernels(m1lya_batch* b, int parity)
         BProf p(b, "k_step");
         // the variant with the reference-row path for sheared / anisotropic velocity gradients only
         // when a cell of the batch needs it: it costs registers the common path does not pay for
+        const size_t lds = step_lds_bytes(b->wpb);
         if (b->shear)
-            hipLaunchKernelGGL(k_step<true>, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs,
+            hipLaunchKernelGGL(k_step<true>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
                                b->nblk, b->wpb, parity, b->krange, b->part);
         else
-            hipLaunchKernelGGL(k_step<false>, dim3(b->grid), dim3(64 * b->wpb), 0, s, b->d_cl, b->A, b->d_segs,
+            hipLaunchKernelGGL(k_step<false>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
                                b->nblk, b->wpb, parity, b->krange, b->part);
         p.done();
     }
@@ -1807,7 +1820,7 @@ int m1lya_batch_set_tiles(m1lya_batch* b, int seg_len, int waves_per_block)
     return bguard(b, [&] {
         if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
         if (seg_len < 1 || seg_len > 4096) bfail(b, M1LYA_ERR_ARG, "seg_len out of range [1, 4096]");
-        if (waves_per_block < 1 || waves_per_block > kMaxWaves) bfail(b, M1LYA_ERR_ARG, "waves_per_block must be 1..4");
+        if (waves_per_block < 1 || waves_per_block > kMaxWaves) bfail(b, M1LYA_ERR_ARG, "waves_per_block must be 1..8");
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         b->seg_len = seg_len;
         b->wpb = waves_per_block;
@@ -1871,9 +1884,10 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         set_ctrl(b, nullptr, 0);
         // segment lengths: powers of two (8 gives 4 wavefronts per SIMD, the kernel's
         // occupancy, on 128 rows of 256 cells) and the lengths that give 3 or 6 wavefronts
-        // per SIMD on 256-cell rows; 2 or 4 wavefronts per block
+        // per SIMD on 256-cell rows; 2, 4 or 8 wavefronts per block (taller blocks share
+        // more Z interfaces and halo rows, shorter ones fit more blocks per CU)
         static const int seg_opts[] = {6, 8, 11, 16, 22, 32, 43, 64};
-        static const int wpb_opts[] = {2, 4};
+        static const int wpb_opts[] = {2, 4, 8};
         int best_s = b->seg_len, best_w = b->wpb;
         double best = 1e300;
         hipEvent_t e0, e1;

@@ -161,7 +161,7 @@ def test_tile_shape_does_not_change_state(lut):
     ref = M1LyaBatch(clouds)
     ref.set_tiles(16, 4)
     h_ref = ref.run_steps(4)
-    for tiles in [(4, 1), (7, 2), (64, 3)]:
+    for tiles in [(4, 1), (7, 2), (64, 3), (16, 8), (5, 6)]:
         b = M1LyaBatch(clouds)
         b.set_tiles(*tiles)
         h = b.run_steps(4)
