@@ -158,73 +158,83 @@ __device__ __forceinline__ void load_z(const SpecIn& s, int nn, int nr, int nz, 
 
 // ------------------------------------------------------------------------
 // AdvectJandH_Cylindrical!: src/AdvectJandH.jl:526-1390.
-// One thread per (k, cell); each thread evaluates the four interfaces of its
-// cell and applies the cell update (:1302-1382).  Reads the state, writes the
-// new state to separate buffers (all fluxes of a step are computed from the
-// same old state, as in the reference).  Bins outside [kmin, kmax] are copied.
+// A block is 64 bins x BI cells along r x BJ cells along Z, one wavefront per
+// cell.  Each interface is evaluated once: a wavefront evaluates its cell's left
+// r face and bottom Z face, the block's last column and row (or the cloud's last
+// cell) also the right / top face, and the results go through LDS to the two
+// cells that share them.  Then each thread applies its cell update (:1302-1382).
+// Reads the state, writes the new state to separate buffers (all fluxes of a
+// step are computed from the same old state, as in the reference).  Bins outside
+// [kmin, kmax] are copied.  A face's value does not depend on which cell's
+// wavefront evaluates it, so the result is the one of a per-cell evaluation.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_advect(AdvectArgs a)
+template <int BI, int BJ>
+__global__ __launch_bounds__(64 * BI * BJ) void k_advect(AdvectArgs a)
 {
+    // r faces of the block's rows: F_J, F_Hr, F_HZ, K_ϕϕ left, K_ϕϕ right (the inner-axis
+    // value at f = 0); Z faces of its columns: F_J, F_Hr, F_HZ
+    __shared__ double rF[BJ][BI + 1][5][64];
+    __shared__ double zF[BJ + 1][BI][3][64];
     const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int i = blockIdx.y * 4 + threadIdx.y;
-    const int j = blockIdx.z;
-    if (k >= nn || i >= nr) return;
-    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    const int lane = threadIdx.x, il = threadIdx.y, jl = threadIdx.z;
+    const int k = blockIdx.x * 64 + lane;
+    const int i = blockIdx.y * BI + il;
+    const int j = blockIdx.z * BJ + jl;
+    const bool cell_ok = k < nn && i < nr && j < nz;
     const int kmin = a.krange[0], kmax = a.krange[1];
+    const bool act = cell_ok && k >= kmin && k <= kmax;
     const SpecIn s{a.J, a.Hr, a.Hz};
-    if (k < kmin || k > kmax) {
+
+    auto r_face = [&](int f, int slot) {
+        Stencil st;
+        const int cl = f - 1;
+        const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) load_r(s, nn, nr, j, k, cells[m], a.bc_r_right, st.J[m], st.Hr[m], st.Hz[m]);
+        const double aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
+        const double aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
+        const double u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
+        const FaceResult r = interface_flux(0, a.hll, st, a.g.ivr + (size_t)f * 5, aFL, aFR, u);
+        double* o = &rF[jl][slot][0][lane];
+        o[0] = r.F.J; o[64] = r.F.Hr; o[128] = r.F.Hz;
+        o[192] = r.kphi_L;
+        o[256] = (f == 0) ? kphi_inner_axis(r.J_R, r.Hz_R) : r.kphi_R;
+    };
+    auto z_face = [&](int g, int slot) {
+        Stencil st;
+        const int cl = g - 1;
+        const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            load_z(s, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, st.J[m], st.Hr[m], st.Hz[m]);
+        const double aFL = a.aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
+        const double aFR = a.aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+        const double u = a.v.uz_star[(size_t)g * nr + i] * kInvC;
+        const FaceResult r = interface_flux(1, a.hll, st, a.g.ivz + (size_t)g * 5, aFL, aFR, u);
+        double* o = &zF[slot][il][0][lane];
+        o[0] = r.F.J; o[64] = r.F.Hr; o[128] = r.F.Hz;
+    };
+    if (act) {
+        r_face(i, il);
+        if (il == BI - 1 || i == nr - 1) r_face(i + 1, il + 1);
+        z_face(j, jl);
+        if (jl == BJ - 1 || j == nz - 1) z_face(j + 1, jl + 1);
+    }
+    __syncthreads();
+    if (!cell_ok) return;
+    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    if (!act) {
         a.J_out[idx] = s.J[idx];
         a.Hr_out[idx] = s.Hr[idx];
         a.Hz_out[idx] = s.Hz[idx];
         return;
     }
-    double FrL0 = 0.0, FrL1 = 0.0, FrL2 = 0.0, FrR0 = 0.0, FrR1 = 0.0, FrR2 = 0.0;
-    double FzL0 = 0.0, FzL1 = 0.0, FzL2 = 0.0, FzR0 = 0.0, FzR1 = 0.0, FzR2 = 0.0;
-    double kphi_im = 0.0, kphi_ip = 0.0;
-#pragma unroll 1
-    for (int side = 0; side < 4; ++side) {
-        Stencil st;
-        const double* iv;
-        double aFL, aFR, u;
-        int dir;
-        if (side < 2) {
-            dir = 0;
-            const int f = i + side;   // face index: left face of cell f
-            const int cl = f - 1;
-            const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
-#pragma unroll
-            for (int m = 0; m < 4; ++m) load_r(s, nn, nr, j, k, cells[m], a.bc_r_right, st.J[m], st.Hr[m], st.Hz[m]);
-            iv = a.g.ivr + (size_t)f * 5;
-            aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
-            aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
-            u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
-        } else {
-            dir = 1;
-            const int g = j + (side - 2);
-            const int cl = g - 1;
-            const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-                load_z(s, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, st.J[m], st.Hr[m], st.Hz[m]);
-            iv = a.g.ivz + (size_t)g * 5;
-            aFL = a.aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
-            aFR = a.aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
-            u = a.v.uz_star[(size_t)g * nr + i] * kInvC;
-        }
-        FaceResult r = interface_flux(dir, a.hll, st, iv, aFL, aFR, u);
-        if (side == 0) {
-            FrL0 = r.F.J; FrL1 = r.F.Hr; FrL2 = r.F.Hz;
-            kphi_im = (i == 0) ? kphi_inner_axis(r.J_R, r.Hz_R) : r.kphi_R;
-        } else if (side == 1) {
-            FrR0 = r.F.J; FrR1 = r.F.Hr; FrR2 = r.F.Hz;
-            kphi_ip = r.kphi_L;
-        } else if (side == 2) {
-            FzL0 = r.F.J; FzL1 = r.F.Hr; FzL2 = r.F.Hz;
-        } else {
-            FzR0 = r.F.J; FzR1 = r.F.Hr; FzR2 = r.F.Hz;
-        }
-    }
+    const double FrL0 = rF[jl][il][0][lane], FrL1 = rF[jl][il][1][lane], FrL2 = rF[jl][il][2][lane];
+    const double FrR0 = rF[jl][il + 1][0][lane], FrR1 = rF[jl][il + 1][1][lane], FrR2 = rF[jl][il + 1][2][lane];
+    const double FzL0 = zF[jl][il][0][lane], FzL1 = zF[jl][il][1][lane], FzL2 = zF[jl][il][2][lane];
+    const double FzR0 = zF[jl + 1][il][0][lane], FzR1 = zF[jl + 1][il][1][lane], FzR2 = zF[jl + 1][il][2][lane];
+    const double kphi_im = rF[jl][il][4][lane];
+    const double kphi_ip = rF[jl][il + 1][3][lane];
 
     // ---- cell update: src/AdvectJandH.jl:1319-1379 ----
     const int c = j * nr + i;
@@ -763,8 +773,10 @@ void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, in
 
 void launch_advect(hipStream_t s, const AdvectArgs& a)
 {
-    dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
-    hipLaunchKernelGGL(k_advect, grid, dim3(64, 4), 0, s, a);
+    // 2 x 4 cells: 2.75 face evaluations per cell instead of 4, 46 KB of LDS, two blocks per CU
+    constexpr int BI = 2, BJ = 4;
+    dim3 grid((a.nn + 63) / 64, (a.g.nr + BI - 1) / BI, (a.g.nz + BJ - 1) / BJ);
+    hipLaunchKernelGGL((k_advect<BI, BJ>), grid, dim3(64, BI, BJ), 0, s, a);
 }
 
 template <int W, int KPL>
