@@ -25,6 +25,17 @@ struct Dl {
     __device__ Dl(double x, double dx) : v(x), d(dx) {}
 };
 
+// 1/x for tangents: the hardware estimate and two Newton steps (a few ulp).  Values are
+// formed exactly as the forward forms them; only their derivatives use this.
+__device__ __forceinline__ double trcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 __device__ __forceinline__ Dl operator+(Dl a, Dl b) { return Dl(a.v + b.v, a.d + b.d); }
 __device__ __forceinline__ Dl operator-(Dl a, Dl b) { return Dl(a.v - b.v, a.d - b.d); }
 __device__ __forceinline__ Dl operator-(Dl a) { return Dl(-a.v, -a.d); }
@@ -32,7 +43,7 @@ __device__ __forceinline__ Dl operator*(Dl a, Dl b) { return Dl(a.v * b.v, a.d *
 __device__ __forceinline__ Dl operator/(Dl a, Dl b)
 {
     double q = a.v / b.v;
-    return Dl(q, (a.d - q * b.d) / b.v);
+    return Dl(q, (a.d - q * b.d) * trcp(b.v));
 }
 __device__ __forceinline__ Dl operator+(Dl a, double b) { return Dl(a.v + b, a.d); }
 __device__ __forceinline__ Dl operator+(double a, Dl b) { return Dl(a + b.v, b.d); }
@@ -40,11 +51,11 @@ __device__ __forceinline__ Dl operator-(Dl a, double b) { return Dl(a.v - b, a.d
 __device__ __forceinline__ Dl operator-(double a, Dl b) { return Dl(a - b.v, -b.d); }
 __device__ __forceinline__ Dl operator*(Dl a, double b) { return Dl(a.v * b, a.d * b); }
 __device__ __forceinline__ Dl operator*(double a, Dl b) { return Dl(a * b.v, a * b.d); }
-__device__ __forceinline__ Dl operator/(Dl a, double b) { return Dl(a.v / b, a.d / b); }
+__device__ __forceinline__ Dl operator/(Dl a, double b) { return Dl(a.v / b, a.d * trcp(b)); }
 __device__ __forceinline__ Dl operator/(double a, Dl b)
 {
     double q = a / b.v;
-    return Dl(q, -q * b.d / b.v);
+    return Dl(q, -q * b.d * trcp(b.v));
 }
 __device__ __forceinline__ Dl& operator+=(Dl& a, Dl b) { a = a + b; return a; }
 __device__ __forceinline__ Dl& operator*=(Dl& a, Dl b) { a = a * b; return a; }
@@ -58,7 +69,7 @@ __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ Dl tsqrt(Dl x)
 {
     double s = sqrt(x.v);
-    return Dl(s, s > 0.0 ? 0.5 * x.d / s : 0.0);
+    return Dl(s, s > 0.0 ? 0.5 * x.d * trcp(s) : 0.0);
 }
 __device__ __forceinline__ double texp(double x) { return exp_det(x); }
 __device__ __forceinline__ Dl texp(Dl x)
