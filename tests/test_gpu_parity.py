@@ -118,11 +118,16 @@ def test_reductions_match(lut):
     assert rel(ng, no) < RED_TOL
 
 
-@pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow"])
+@pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow", "odd_grid_outflow"])
 def test_fused_step_bit_exact(lut, case):
     if case == "static_cloud_32x32x64":
         cloud = L.static_cloud_32x32x64(lut=lut)
         nsteps = 40
+    elif case == "odd_grid_outflow":
+        # N_r and N_Z not multiples of the advection block (2 x 4 cells), open bottom
+        cloud = randomize(small_cloud(n_r=13, n_z=11, n_nu=48, tau0=1e7, tau_abs=1.0, umax=100.0,
+                                      bc_z_bot=L.OUTFLOW, lut=lut), seed=16)
+        nsteps = 8
     else:
         cloud = randomize(small_cloud(n_r=24, n_z=20, n_nu=128, tau0=1e8, tau_abs=1.0, theta_cone=np.pi / 4,
                                       umax=200.0, lut=lut), seed=15)
