@@ -143,20 +143,28 @@ def run_fp64(cloud, steps, warmup):
 
 def run_fp32(clouds, steps, warmup, autotune):
     """Timed batched FP32 steps of a batch of clouds.  The timed region launches the
-    kernels eagerly with HIP events around each one (no host synchronisation); a second,
-    untimed-for-the-headline pass replays the captured hipGraph."""
+    kernels eagerly with HIP events around each k_step launch (the dominant kernel; no
+    host synchronisation).  An untimed pass of up to 20 steps with every launch bracketed
+    gives the small reduction kernels' share, and a last pass replays the captured hipGraph."""
     from lydion_amd.batch import M1LyaBatch
     b = M1LyaBatch(clouds)
     tiles = b.autotune(trial_steps=3) if autotune else (*b.tiles(), None)
     b.run_steps(warmup, history=False)
     b.reset_kernel_stats()
-    b.set_profiling(True)
+    b.set_profiling(2)
     b.synchronize()
     t0 = time.perf_counter()
     b.run_steps(steps, history=False)
     b.synchronize()
     wall = time.perf_counter() - t0
     stats = b.kernel_stats()
+    nb = min(steps, 20)
+    b.reset_kernel_stats()
+    b.set_profiling(1)
+    b.run_steps(nb, history=False)
+    for name, (launches, total_ms) in b.kernel_stats().items():
+        if name not in stats:   # per-step share scaled to the timed steps
+            stats[name] = (launches * steps // nb, total_ms * steps / nb)
     b.set_profiling(False)
     b.set_graph(True)
     b.run_steps(2, history=False)

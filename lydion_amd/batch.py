@@ -205,7 +205,8 @@ class M1LyaBatch:
                                                             int(bool(phi_all_bins))))
 
     # ---- profiling ----
-    def set_profiling(self, on: bool) -> None:
+    def set_profiling(self, on) -> None:
+        """True / 1: time every launch; 2: time k_step only; False / 0: off."""
         self._check(self.lib.m1lya_batch_set_profiling(self.handle, int(on)))
 
     def kernel_stats(self) -> dict:

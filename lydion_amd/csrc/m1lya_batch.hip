@@ -1060,7 +1060,7 @@ struct m1lya_batch {
 
     std::vector<void*> allocs;
     std::string err;
-    bool profiling = false;
+    int profiling = 0;                     // 0 off, 1 every launch, 2 k_step only
     std::map<std::string, KStatB> stats;
     struct Pending {
         const char* name;
@@ -1207,10 +1207,12 @@ hipEvent_t pool_event(m1lya_batch* b)
 struct BProf {
     m1lya_batch* b;
     const char* name;
+    bool on;
     hipEvent_t e0{}, e1{};
-    BProf(m1lya_batch* b_, const char* n) : b(b_), name(n)
+    BProf(m1lya_batch* b_, const char* n, bool step_kernel = false)
+        : b(b_), name(n), on(b_->profiling == 1 || (b_->profiling == 2 && step_kernel))
     {
-        if (b->profiling) {
+        if (on) {
             e0 = pool_event(b);
             e1 = pool_event(b);
             bchk(b, hipEventRecord(e0, b->stream), "hipEventRecord");
@@ -1219,7 +1221,7 @@ struct BProf {
     void done()
     {
         bchk(b, hipGetLastError(), name);
-        if (b->profiling) {
+        if (on) {
             bchk(b, hipEventRecord(e1, b->stream), "hipEventRecord");
             b->pending.push_back(m1lya_batch::Pending{name, e0, e1});
         }
@@ -1246,7 +1248,7 @@ void launch_step_kernels(m1lya_batch* b, int parity)
 {
     hipStream_t s = b->stream;
     {
-        BProf p(b, "k_step");
+        BProf p(b, "k_step", true);
         // the variant with the reference-row path for sheared / anisotropic velocity gradients only
         // when a cell of the batch needs it: it costs registers the common path does not pay for
         const size_t lds = step_lds_bytes(b->wpb);
@@ -1299,8 +1301,8 @@ void require_ready(m1lya_batch* b)
 
 void capture_graphs(m1lya_batch* b)
 {
-    const bool prof = b->profiling;
-    b->profiling = false;
+    const int prof = b->profiling;
+    b->profiling = 0;
     for (int p = 0; p < 2; ++p) {
         if (b->gexec[p]) continue;
         hipGraph_t g;
@@ -1878,8 +1880,8 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
             copy(b->krange, sK, b->n * 2 * sizeof(int));
             b->parity = p0;
         };
-        const bool prof = b->profiling;
-        b->profiling = false;
+        const int prof = b->profiling;
+        b->profiling = 0;
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         set_ctrl(b, nullptr, 0);
         // segment lengths: powers of two (8 gives 4 wavefronts per SIMD, the kernel's
@@ -1948,7 +1950,7 @@ int m1lya_batch_synchronize(m1lya_batch* b)
 int m1lya_batch_set_profiling(m1lya_batch* b, int on)
 {
     if (!b) return M1LYA_ERR_ARG;
-    b->profiling = on != 0;
+    b->profiling = (on == 2) ? 2 : (on != 0 ? 1 : 0);
     return M1LYA_OK;
 }
 

@@ -259,3 +259,19 @@ def test_rhd_like_state_vs_fp64(lut):
     assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
     ctx.close()
     b.close()
+
+
+def test_profiling_modes(lut):
+    """Profiling level 2 brackets only k_step (two events per step); level 1 every launch."""
+    b = M1LyaBatch(_mixed_clouds(lut)[:1])
+    b.set_profiling(2)
+    b.run_steps(3, history=False)
+    st = b.kernel_stats()
+    assert set(st) == {"k_step"} and st["k_step"][0] == 3
+    b.reset_kernel_stats()
+    b.set_profiling(1)
+    b.run_steps(2, history=False)
+    st = b.kernel_stats()
+    assert {"k_step", "k_reduce", "k_final"} <= set(st) and all(v[0] == 2 for k, v in st.items() if k in ("k_step", "k_reduce", "k_final"))
+    b.set_profiling(0)
+    b.close()
