@@ -60,8 +60,48 @@ __device__ __forceinline__ Dl operator/(double a, Dl b)
 __device__ __forceinline__ Dl& operator+=(Dl& a, Dl b) { a = a + b; return a; }
 __device__ __forceinline__ Dl& operator*=(Dl& a, Dl b) { a = a * b; return a; }
 
+// Value and two tangents: two directional derivatives in one pass.
+struct Dl2 {
+    double v, d0, d1;
+    __device__ Dl2() : v(0.0), d0(0.0), d1(0.0) {}
+    __device__ Dl2(double x) : v(x), d0(0.0), d1(0.0) {}
+    __device__ Dl2(double x, double a, double b) : v(x), d0(a), d1(b) {}
+};
+
+__device__ __forceinline__ Dl2 operator+(Dl2 a, Dl2 b) { return Dl2(a.v + b.v, a.d0 + b.d0, a.d1 + b.d1); }
+__device__ __forceinline__ Dl2 operator-(Dl2 a, Dl2 b) { return Dl2(a.v - b.v, a.d0 - b.d0, a.d1 - b.d1); }
+__device__ __forceinline__ Dl2 operator-(Dl2 a) { return Dl2(-a.v, -a.d0, -a.d1); }
+__device__ __forceinline__ Dl2 operator*(Dl2 a, Dl2 b)
+{
+    return Dl2(a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1);
+}
+__device__ __forceinline__ Dl2 operator/(Dl2 a, Dl2 b)
+{
+    const double q = a.v / b.v, r = trcp(b.v);
+    return Dl2(q, (a.d0 - q * b.d0) * r, (a.d1 - q * b.d1) * r);
+}
+__device__ __forceinline__ Dl2 operator+(Dl2 a, double b) { return Dl2(a.v + b, a.d0, a.d1); }
+__device__ __forceinline__ Dl2 operator+(double a, Dl2 b) { return Dl2(a + b.v, b.d0, b.d1); }
+__device__ __forceinline__ Dl2 operator-(Dl2 a, double b) { return Dl2(a.v - b, a.d0, a.d1); }
+__device__ __forceinline__ Dl2 operator-(double a, Dl2 b) { return Dl2(a - b.v, -b.d0, -b.d1); }
+__device__ __forceinline__ Dl2 operator*(Dl2 a, double b) { return Dl2(a.v * b, a.d0 * b, a.d1 * b); }
+__device__ __forceinline__ Dl2 operator*(double a, Dl2 b) { return Dl2(a * b.v, a * b.d0, a * b.d1); }
+__device__ __forceinline__ Dl2 operator/(Dl2 a, double b)
+{
+    const double r = trcp(b);
+    return Dl2(a.v / b, a.d0 * r, a.d1 * r);
+}
+__device__ __forceinline__ Dl2 operator/(double a, Dl2 b)
+{
+    const double q = a / b.v, r = -q * trcp(b.v);
+    return Dl2(q, r * b.d0, r * b.d1);
+}
+__device__ __forceinline__ Dl2& operator+=(Dl2& a, Dl2 b) { a = a + b; return a; }
+__device__ __forceinline__ Dl2& operator*=(Dl2& a, Dl2 b) { a = a * b; return a; }
+
 __device__ __forceinline__ double val(double x) { return x; }
 __device__ __forceinline__ double val(const Dl& x) { return x.v; }
+__device__ __forceinline__ double val(const Dl2& x) { return x.v; }
 __device__ __forceinline__ double der(double) { return 0.0; }
 __device__ __forceinline__ double der(const Dl& x) { return x.d; }
 
@@ -71,14 +111,26 @@ __device__ __forceinline__ Dl tsqrt(Dl x)
     double s = sqrt(x.v);
     return Dl(s, s > 0.0 ? 0.5 * x.d * trcp(s) : 0.0);
 }
+__device__ __forceinline__ Dl2 tsqrt(Dl2 x)
+{
+    const double s = sqrt(x.v);
+    const double h = s > 0.0 ? 0.5 * trcp(s) : 0.0;
+    return Dl2(s, h * x.d0, h * x.d1);
+}
 __device__ __forceinline__ double texp(double x) { return exp_det(x); }
 __device__ __forceinline__ Dl texp(Dl x)
 {
     double e = exp_det(x.v);
     return Dl(e, e * x.d);
 }
+__device__ __forceinline__ Dl2 texp(Dl2 x)
+{
+    const double e = exp_det(x.v);
+    return Dl2(e, e * x.d0, e * x.d1);
+}
 __device__ __forceinline__ double tabs(double x) { return fabs(x); }
 __device__ __forceinline__ Dl tabs(Dl x) { return x.v < 0.0 ? -x : x; }
+__device__ __forceinline__ Dl2 tabs(Dl2 x) { return x.v < 0.0 ? -x : x; }
 
 // max/min/clamp with the forward's tie rules (m1lya_device.h dmax/dmin/dclamp)
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return (val(b) > val(a)) ? b : a; }

@@ -37,11 +37,22 @@
 namespace m1 {
 
 using m1a::Dl;
+using m1a::Dl2;
 
 namespace {
 
-__device__ __forceinline__ void seed(double&, double) {}
-__device__ __forceinline__ void seed(Dl& x, double d) { x.d = d; }
+// Tangent d on input `which` (0 J, 1 H_r, 2 H_Z, 3 α_F of the target cell) when it is
+// the direction `var` asks for: Dl carries direction var; Dl2 the pair (2 var, 2 var + 1).
+__device__ __forceinline__ void seedv(double&, int, int, double) {}
+__device__ __forceinline__ void seedv(Dl& x, int which, int var, double d)
+{
+    if (which == var) x.d = d;
+}
+__device__ __forceinline__ void seedv(Dl2& x, int which, int var, double d)
+{
+    if (which == 2 * var) x.d0 = d;
+    else if (which == 2 * var + 1) x.d1 = d;
+}
 
 // Ghost-aware stencil load along r (the values of load_r in m1lya_kernels.hip),
 // with the tangent of the slot along variable `var` (0 J, 1 H_r, 2 H_Z, else
@@ -63,9 +74,9 @@ __device__ __forceinline__ void tload_r(const double* J, const double* Hr, const
     sHr = T(clip ? dmax(vHr, 0.0) : hr_fac * vHr);
     sHz = T(vHz);
     if (src == tc) {
-        if (var == 0) seed(sJ, 1.0);
-        else if (var == 1) seed(sHr, clip ? ((0.0 > vHr) ? 0.0 : 1.0) : hr_fac);
-        else if (var == 2) seed(sHz, 1.0);
+        seedv(sJ, 0, var, 1.0);
+        seedv(sHr, 1, var, clip ? ((0.0 > vHr) ? 0.0 : 1.0) : hr_fac);
+        seedv(sHz, 2, var, 1.0);
     }
 }
 
@@ -97,9 +108,9 @@ __device__ __forceinline__ void tload_z(const double* J, const double* Hr, const
     if (clip == 2) { hz = dmin(vHz, 0.0); dhz = (0.0 < vHz) ? 0.0 : 1.0; }
     sHz = T(hz);
     if (src == tc) {
-        if (var == 0) seed(sJ, 1.0);
-        else if (var == 1) seed(sHr, 1.0);
-        else if (var == 2) seed(sHz, dhz);
+        seedv(sJ, 0, var, 1.0);
+        seedv(sHr, 1, var, 1.0);
+        seedv(sHz, 2, var, dhz);
     }
 }
 
@@ -122,10 +133,8 @@ __device__ m1a::FaceT<T> face_at(const AdjAdvArgs& a, int dir, int f, int i, int
         const int cl = max(f - 1, 0), cr = min(f, nr - 1);
         aFL = T(a.aF[((size_t)j * nr + cl) * nn + k]);
         aFR = T(a.aF[((size_t)j * nr + cr) * nn + k]);
-        if (var == 3) {
-            if (cl == tc) seed(aFL, 1.0);
-            if (cr == tc) seed(aFR, 1.0);
-        }
+        if (cl == tc) seedv(aFL, 3, var, 1.0);
+        if (cr == tc) seedv(aFR, 3, var, 1.0);
         u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
     } else {
 #pragma unroll
@@ -136,10 +145,8 @@ __device__ m1a::FaceT<T> face_at(const AdjAdvArgs& a, int dir, int f, int i, int
         const int cl = max(f - 1, 0), cr = min(f, nz - 1);
         aFL = T(a.aF[((size_t)cl * nr + i) * nn + k]);
         aFR = T(a.aF[((size_t)cr * nr + i) * nn + k]);
-        if (var == 3) {
-            if (cl == tc) seed(aFL, 1.0);
-            if (cr == tc) seed(aFR, 1.0);
-        }
+        if (cl == tc) seedv(aFL, 3, var, 1.0);
+        if (cr == tc) seedv(aFR, 3, var, 1.0);
         u = a.v.uz_star[(size_t)f * nr + i] * kInvC;
     }
     return m1a::tface<T>(dir, a.hll, dir == 0 && f == 0, st, iv, aFL, aFR, u);
@@ -244,7 +251,8 @@ __global__ __launch_bounds__(256) void k_adj_cell(AdjAdvArgs a)
 // ---------------------------------------------------------------------------
 // Interface contributions gathered per cell.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_adj_gather(AdjAdvArgs a)
+// two waves per SIMD: the Dl2 passes need about 280 VGPRs unbounded, 256 with a few spills
+__global__ __launch_bounds__(256, 2) void k_adj_gather(AdjAdvArgs a)
 {
     const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
     const int k = blockIdx.x * 64 + threadIdx.x;
@@ -281,10 +289,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjAdvArgs a)
         }
         if (lF[0] == 0.0 && lF[1] == 0.0 && lF[2] == 0.0 && lkL == 0.0 && lkR == 0.0) continue;
         const int tc = dir == 0 ? i : j;
+        // two directions per pass: (J, H_r), then (H_Z, α_F)
 #pragma unroll 1
-        for (int var = 0; var < 4; ++var) {
-            m1a::FaceT<Dl> r = face_at<Dl>(a, dir, f, i, j, k, tc, var);
-            acc[var] += lF[0] * r.FJ.d + lF[1] * r.FHr.d + lF[2] * r.FHz.d + lkL * r.kphi_L.d + lkR * r.kphi_R.d;
+        for (int vp = 0; vp < 2; ++vp) {
+            m1a::FaceT<Dl2> r = face_at<Dl2>(a, dir, f, i, j, k, tc, vp);
+            acc[2 * vp] += lF[0] * r.FJ.d0 + lF[1] * r.FHr.d0 + lF[2] * r.FHz.d0 + lkL * r.kphi_L.d0 + lkR * r.kphi_R.d0;
+            acc[2 * vp + 1] +=
+                lF[0] * r.FJ.d1 + lF[1] * r.FHr.d1 + lF[2] * r.FHz.d1 + lkL * r.kphi_L.d1 + lkR * r.kphi_R.d1;
         }
     }
     a.oJ[idx] += acc[0];
