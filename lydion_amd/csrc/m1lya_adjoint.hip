@@ -486,15 +486,18 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
         const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
         const m1a::RowConst rc = row_const(k);
+        // derivatives along the six inputs, two per pass
         double g[6];
 #pragma unroll 1
-        for (int d = 0; d < 6; ++d) {
-            if (d == 4 && !a.recoil) { g[d] = 0.0; continue; }
-            Dl hm(Hv[k - 1], d == 0 ? 1.0 : 0.0), h0(Hv[k], d == 1 ? 1.0 : 0.0), hp(Hv[k + 1], d == 2 ? 1.0 : 0.0);
-            Dl pf(PREFACT, d == 3 ? 1.0 : 0.0), rec(RECOIL_0, d == 4 ? 1.0 : 0.0), vel(VEL, d == 5 ? 1.0 : 0.0);
-            m1a::RowT<Dl> row = m1a::tfreq_row<Dl>(rc, hm, h0, hp, pf, rec, vel);
-            g[d] = lFkp * row.Fkp.d + lFk * row.Fk.d + lFkm * row.Fkm.d;
+        for (int dp = 0; dp < 3; ++dp) {
+            auto in = [&](double v, int d) { return Dl2(v, d == 2 * dp ? 1.0 : 0.0, d == 2 * dp + 1 ? 1.0 : 0.0); };
+            const Dl2 hm = in(Hv[k - 1], 0), h0 = in(Hv[k], 1), hp = in(Hv[k + 1], 2);
+            const Dl2 pf = in(PREFACT, 3), rec = in(RECOIL_0, 4), vel = in(VEL, 5);
+            m1a::RowT<Dl2> row = m1a::tfreq_row<Dl2>(rc, hm, h0, hp, pf, rec, vel);
+            g[2 * dp] = lFkp * row.Fkp.d0 + lFk * row.Fk.d0 + lFkm * row.Fkm.d0;
+            g[2 * dp + 1] = lFkp * row.Fkp.d1 + lFk * row.Fk.d1 + lFkm * row.Fkm.d1;
         }
+        if (!a.recoil) g[4] = 0.0;
         gm[k] = g[0];
         g0[k] = g[1];
         gp[k] = g[2];
@@ -502,16 +505,13 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         lREC += g[4];
         const double lDdu = g[5] * kInvC;
         if (lDdu != 0.0) {
-#pragma unroll 1
-            for (int var = 0; var < 3; ++var) {
-                Dl tJ(X2J[k], var == 0 ? 1.0 : 0.0), tHr(X2r[k], var == 1 ? 1.0 : 0.0);
-                Dl tHz(X2z[k], var == 2 ? 1.0 : 0.0);
-                m1a::EddT<Dl> Dd = m1a::tD_edd(tHr, tHz, tJ);
-                const double dd = m1a::tD_colon_grad_u(Dd, durdr, durdz, duzdr, duzdz, urr).d;
-                if (var == 0) lJ[k] += lDdu * dd;
-                else if (var == 1) lHr[k] += lDdu * dd;
-                else lHz[k] += lDdu * dd;
-            }
+            // D:∇u along (J, H_r) in one pass, then along H_Z
+            const Dl2 tJ(X2J[k], 1.0, 0.0), tHr(X2r[k], 0.0, 1.0), tHz(X2z[k]);
+            const Dl2 dd2 = m1a::tD_colon_grad_u(m1a::tD_edd(tHr, tHz, tJ), durdr, durdz, duzdr, duzdz, urr);
+            lJ[k] += lDdu * dd2.d0;
+            lHr[k] += lDdu * dd2.d1;
+            const Dl uJ(X2J[k]), uHr(X2r[k]), uHz(X2z[k], 1.0);
+            lHz[k] += lDdu * m1a::tD_colon_grad_u(m1a::tD_edd(uHr, uHz, uJ), durdr, durdz, duzdr, duzdz, urr).d;
         }
     }
     __syncthreads();
