@@ -65,7 +65,7 @@ EXPORTED = [
     "m1lya_update_f_mf_l_gravity", "m1lya_compute_f_esc", "m1lya_reset_drhou", "m1lya_update_e_lya",
     "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_adjoint_run",
     "m1lya_set_voigt_lut", "m1lya_prepare_subcycling",
-    "m1lya_set_profiling",
+    "m1lya_set_profiling", "m1lya_set_adjoint_mode",
     "m1lya_kernel_stats_count", "m1lya_kernel_stats", "m1lya_reset_kernel_stats", "m1lya_selftest_math",
     "m1lya_batch_create", "m1lya_batch_destroy", "m1lya_batch_last_error", "m1lya_batch_count",
     "m1lya_batch_set_geometry", "m1lya_batch_set_freq_grid", "m1lya_batch_set_scale", "m1lya_batch_get_scale",
@@ -121,6 +121,7 @@ def load() -> C.CDLL:
         "m1lya_set_voigt_lut": (i, [vp, _dp, i, d, C.POINTER(LineConstants)]),
         "m1lya_prepare_subcycling": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "m1lya_set_profiling": (i, [vp, i]),
+        "m1lya_set_adjoint_mode": (i, [vp, i]),
         "m1lya_kernel_stats_count": (i, [vp]),
         "m1lya_kernel_stats": (i, [vp, i, C.c_char_p, sz, C.POINTER(C.c_long), _dp]),
         "m1lya_reset_kernel_stats": (i, [vp]),

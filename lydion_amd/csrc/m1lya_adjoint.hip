@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include "m1lya_adjoint.h"
+#include "m1lya_adjoint_face.h"
 #include "m1lya_internal.h"
 
 namespace m1 {
@@ -296,6 +297,167 @@ __global__ __launch_bounds__(256, 2) void k_adj_gather(AdjAdvArgs a)
             acc[2 * vp] += lF[0] * r.FJ.d0 + lF[1] * r.FHr.d0 + lF[2] * r.FHz.d0 + lkL * r.kphi_L.d0 + lkR * r.kphi_R.d0;
             acc[2 * vp + 1] +=
                 lF[0] * r.FJ.d1 + lF[1] * r.FHr.d1 + lF[2] * r.FHz.d1 + lkL * r.kphi_L.d1 + lkR * r.kphi_R.d1;
+        }
+    }
+    a.oJ[idx] += acc[0];
+    a.oHr[idx] += acc[1];
+    a.oHz[idx] += acc[2];
+    a.gaF[idx] += acc[3];
+}
+
+// ---------------------------------------------------------------------------
+// Interface contributions through reverse-mode interfaces.  k_adj_face: thread per
+// (bin, interface); the interface cotangents come from the two cells it separates,
+// m1a::face_vjp turns them into cotangents of the interface's 14 inputs (J, H_r, H_Z
+// of its four stencil slots, α_F on its two sides), stored as planes of fpart.
+// k_adj_gather_rev: thread per (bin, cell); sums, over the <= 8 interfaces whose
+// stencil holds the cell, the slots that read it, through the ghost map of each
+// boundary (mirror: -1 on the normal flux; outflow clip: 0 or 1), and the α_F sides
+// that are the cell.  Same contributions as k_adj_gather's dual-number passes, in a
+// fixed order, without atomics.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FacePlanes {
+    size_t base, plane;   // plane p of interface index q at fpart[base + p * plane + q * nn + k]
+};
+
+__device__ __forceinline__ FacePlanes face_planes(int dir, int nn, int nr, int nz)
+{
+    const size_t nfr = (size_t)nz * (nr + 1) * nn, nfz = (size_t)nr * (nz + 1) * nn;
+    return dir == 0 ? FacePlanes{0, nfr} : FacePlanes{14 * nfr, nfz};
+}
+
+// Ghost map of stencil cell c along one direction (as tload_r / tload_z): the source
+// cell, and in fac the derivative of the slot's normal flux component (H_r along r,
+// H_Z along Z) with respect to the source's; J and the other component map with 1.
+// vnorm: the source's normal component, which decides the outflow clips.
+__device__ __forceinline__ int ghost_src(int c, int n, int dir, int low_reflect, int high_reflect, double vnorm,
+                                         double& fac)
+{
+    fac = 1.0;
+    if (c < 0) {
+        if (dir == 0 || low_reflect) { fac = -1.0; return -c - 1; }
+        fac = (0.0 < vnorm) ? 0.0 : 1.0;   // min(H_Z, 0) at an open bottom
+        return 0;
+    }
+    if (c < n) return c;
+    if (high_reflect) { fac = -1.0; return 2 * n - 1 - c; }
+    fac = (0.0 > vnorm) ? 0.0 : 1.0;       // max(H, 0) at an open outer / top boundary
+    return n - 1;
+}
+
+}  // namespace
+
+template <int DIR>
+__global__ __launch_bounds__(256, 2) void k_adj_face(AdjAdvArgs a)
+{
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    // DIR 0: interface f = 0..N_r of row j; DIR 1: interface f = 0..N_Z of column i
+    const int f = DIR == 0 ? (int)(blockIdx.y * 4 + threadIdx.y) : (int)blockIdx.z;
+    const int i = DIR == 0 ? 0 : (int)(blockIdx.y * 4 + threadIdx.y);
+    const int j = DIR == 0 ? (int)blockIdx.z : 0;
+    const int nf = DIR == 0 ? nr : nz;
+    if (k >= nn || f > nf || (DIR == 1 && i >= nr)) return;
+    if (k < a.krange[0] || k > a.krange[1]) return;
+    const double cdt = a.cdt;
+    double lF[3] = {0.0, 0.0, 0.0}, lkL = 0.0, lkR = 0.0;
+    if (f >= 1) {
+        const int cl = DIR == 0 ? j * nr + (f - 1) : (f - 1) * nr + i;
+        const double A = DIR == 0 ? a.g.A_r[(size_t)j * (nr + 1) + f] : a.g.A_z[(size_t)f * nr + i];
+        const double s = -cdt * a.g.inv_dV[cl] * A;
+        const size_t q = (size_t)cl * nn + k;
+        lF[0] += s * a.pJ[q]; lF[1] += s * a.pHr[q]; lF[2] += s * a.pHz[q];
+        if (DIR == 0) lkL = 0.5 * cdt * a.g.inv_r_cell[f - 1] * a.pHr[q];
+    }
+    if (f <= nf - 1) {
+        const int cr = DIR == 0 ? j * nr + f : f * nr + i;
+        const double A = DIR == 0 ? a.g.A_r[(size_t)j * (nr + 1) + f] : a.g.A_z[(size_t)f * nr + i];
+        const double s = cdt * a.g.inv_dV[cr] * A;
+        const size_t q = (size_t)cr * nn + k;
+        lF[0] += s * a.pJ[q]; lF[1] += s * a.pHr[q]; lF[2] += s * a.pHz[q];
+        if (DIR == 0) lkR = 0.5 * cdt * a.g.inv_r_cell[f] * a.pHr[q];
+    }
+    const FacePlanes P = face_planes(DIR, nn, nr, nz);
+    const size_t q0 = P.base + (size_t)(DIR == 0 ? j * (nr + 1) + f : f * nr + i) * nn + k;
+    m1a::FaceGrad g;
+    if (lF[0] == 0.0 && lF[1] == 0.0 && lF[2] == 0.0 && lkL == 0.0 && lkR == 0.0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { g.J[m] = 0.0; g.Hr[m] = 0.0; g.Hz[m] = 0.0; }
+        g.aF_L = 0.0; g.aF_R = 0.0;
+    } else {
+        double J[4], Hr[4], Hz[4];
+        const int cells[4] = {f - 1, f, f + 1, f - 2};
+        double aFL, aFR, u;
+        const double* iv;
+        if (DIR == 0) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                tload_r<double>(a.J, a.Hr, a.Hz, nn, nr, j, k, cells[m], a.bc_r_right, -1, -1, J[m], Hr[m], Hz[m]);
+            iv = a.g.ivr + (size_t)f * 5;
+            aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
+            aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
+            u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                tload_z<double>(a.J, a.Hr, a.Hz, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, -1, -1, J[m],
+                                Hr[m], Hz[m]);
+            iv = a.g.ivz + (size_t)f * 5;
+            aFL = a.aF[((size_t)max(f - 1, 0) * nr + i) * nn + k];
+            aFR = a.aF[((size_t)min(f, nz - 1) * nr + i) * nn + k];
+            u = a.v.uz_star[(size_t)f * nr + i] * kInvC;
+        }
+        g = m1a::face_vjp(DIR, a.hll, DIR == 0 && f == 0, J, Hr, Hz, iv, aFL, aFR, u, lF[0], lF[1], lF[2], lkL, lkR);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        a.fpart[q0 + (size_t)(3 * m) * P.plane] = g.J[m];
+        a.fpart[q0 + (size_t)(3 * m + 1) * P.plane] = g.Hr[m];
+        a.fpart[q0 + (size_t)(3 * m + 2) * P.plane] = g.Hz[m];
+    }
+    a.fpart[q0 + (size_t)12 * P.plane] = g.aF_L;
+    a.fpart[q0 + (size_t)13 * P.plane] = g.aF_R;
+}
+
+__global__ __launch_bounds__(256) void k_adj_gather_rev(AdjAdvArgs a)
+{
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    const int j = blockIdx.z;
+    if (k >= nn || i >= nr) return;
+    if (k < a.krange[0] || k > a.krange[1]) return;
+    const size_t idx = ((size_t)j * nr + i) * nn + k;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int dir = 0; dir < 2; ++dir) {
+        const FacePlanes P = face_planes(dir, nn, nr, nz);
+        const int tc = dir == 0 ? i : j;
+        const int n = dir == 0 ? nr : nz;
+        const int low_reflect = dir == 0 ? 1 : (a.bc_z_bot == BC_REFLECT);
+        const int high_reflect = dir == 0 ? (a.bc_r_right == BC_REFLECT) : (a.bc_z_top == BC_REFLECT);
+        const double vnorm = dir == 0 ? a.Hr[idx] : a.Hz[idx];
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) {
+            const int f = tc - 1 + t;
+            if (f < 0 || f > n) continue;
+            const size_t q0 = P.base + (size_t)(dir == 0 ? j * (nr + 1) + f : f * nr + i) * nn + k;
+            const int cells[4] = {f - 1, f, f + 1, f - 2};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                double fac;
+                const int src = ghost_src(cells[m], n, dir, low_reflect, high_reflect, vnorm, fac);
+                if (src != tc) continue;
+                acc[0] += a.fpart[q0 + (size_t)(3 * m) * P.plane];
+                const double gr = a.fpart[q0 + (size_t)(3 * m + 1) * P.plane];
+                const double gz = a.fpart[q0 + (size_t)(3 * m + 2) * P.plane];
+                if (dir == 0) { acc[1] += fac * gr; acc[2] += gz; }
+                else { acc[1] += gr; acc[2] += fac * gz; }
+            }
+            if (max(f - 1, 0) == tc) acc[3] += a.fpart[q0 + (size_t)12 * P.plane];
+            if (min(f, n - 1) == tc) acc[3] += a.fpart[q0 + (size_t)13 * P.plane];
         }
     }
     a.oJ[idx] += acc[0];
@@ -627,7 +789,13 @@ void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a)
 {
     dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
     hipLaunchKernelGGL(k_adj_cell, grid, dim3(64, 4), 0, s, a);
-    hipLaunchKernelGGL(k_adj_gather, grid, dim3(64, 4), 0, s, a);
+    if (a.mode == 1) {
+        hipLaunchKernelGGL(k_adj_gather, grid, dim3(64, 4), 0, s, a);
+        return;
+    }
+    hipLaunchKernelGGL(k_adj_face<0>, dim3((a.nn + 63) / 64, (a.g.nr + 1 + 3) / 4, a.g.nz), dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL(k_adj_face<1>, dim3((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz + 1), dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL(k_adj_gather_rev, grid, dim3(64, 4), 0, s, a);
 }
 
 void launch_adj_spectrum(hipStream_t s, const double* Hr, const double* Hz, const double* A_r, const double* A_z,

@@ -44,6 +44,7 @@ struct m1lya_ctx {
     bool old_stale = false;   // after fused steps the *_old arrays equal the current state
     bool fij_stale = false;   // Fij_* not maintained by fused steps
     bool krange_valid = false;
+    int adj_mode = 0;                      // adjoint interfaces: 0 reverse mode, 1 dual-number passes
 
     double *aF = nullptr, *Hv = nullptr, *phi = nullptr;
     double *Fkp = nullptr, *Fk = nullptr, *Fkm = nullptr;
@@ -1000,6 +1001,7 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
         double* pre = mem.get(B3);
         double* gbin = mem.get(3 * n3);
         double* gcell = mem.get(5 * nc);
+        double* fpart = c->adj_mode == 0 ? mem.get(adj_face_part_size(nn, nr, nz)) : nullptr;
         double* dv = mem.get(2 * (size_t)nn);
         hipStream_t st = c->stream;
         auto zero = [&](double* p, size_t n) {
@@ -1113,6 +1115,8 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
                 aa.cdt = c_tilde * dt;
                 aa.bc_r_right = bc_r_right; aa.bc_z_bot = bc_z_bot; aa.bc_z_top = bc_z_top; aa.hll = c->flux;
                 aa.krange = kr + 2 * s;
+                aa.fpart = fpart;
+                aa.mode = c->adj_mode;
                 {
                     Prof p(c, "adj_advect");
                     launch_adj_advect(st, aa);
@@ -1139,6 +1143,14 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
             hipchk(c, hipMemcpyAsync(grad_bin, gbin, 3 * n3 * sizeof(double), hipMemcpyDeviceToHost, st), "grad");
         hipchk(c, hipStreamSynchronize(st), "sync");
         check_err_flag(c, "adjoint");
+    });
+}
+
+int m1lya_set_adjoint_mode(m1lya_ctx* c, int mode)
+{
+    return guarded(c, [&] {
+        if (mode != 0 && mode != 1) fail(c, M1LYA_ERR_ARG, "adjoint mode must be 0 or 1");
+        c->adj_mode = mode;
     });
 }
 

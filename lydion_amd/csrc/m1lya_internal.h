@@ -146,7 +146,15 @@ struct AdjAdvArgs {
     double cdt;
     int bc_r_right, bc_z_bot, bc_z_top, hll;
     const int* krange;
+    double* fpart;        // reverse-mode interfaces: 14 planes of per-face input cotangents
+    int mode;             // 0 interfaces in reverse mode; 1 dual-number passes per cell
 };
+
+// doubles of AdjAdvArgs::fpart for an N_r x N_Z x N_ν grid
+inline size_t adj_face_part_size(int nn, int nr, int nz)
+{
+    return (size_t)14 * ((size_t)nz * (nr + 1) + (size_t)nr * (nz + 1)) * nn;
+}
 
 struct AdjColArgs {
     int nn, nr, nz, recoil;

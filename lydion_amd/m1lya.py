@@ -116,6 +116,11 @@ class M1LyaContext:
     def set_profiling(self, on: bool) -> None:
         self._check(self.lib.m1lya_set_profiling(self.handle, int(on)))
 
+    def set_adjoint_mode(self, mode: int) -> None:
+        """0: interfaces differentiated once each in reverse mode (default); 1: dual-number
+        passes per cell, the cross-check of mode 0."""
+        self._check(self.lib.m1lya_set_adjoint_mode(self.handle, int(mode)))
+
     def kernel_stats(self) -> dict:
         n = self.lib.m1lya_kernel_stats_count(self.handle)
         if n < 0:
