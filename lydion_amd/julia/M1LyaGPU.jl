@@ -19,7 +19,7 @@ module M1LyaGPU
 export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
        Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!,
-       set_voigt_lut!, prepare_subcycling!
+       set_voigt_lut!, prepare_subcycling!, set_adjoint_mode!
 
 const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
 
@@ -299,6 +299,15 @@ function adjoint_run!(Lyα, nsteps::Integer, c̃::Float64, Δt::Float64, Z_cente
         checkpoint_every, obj, hist, gs, gc, gb), ctx)
     return obj[], hist, gs, gc, gb
 end
+
+"""
+    set_adjoint_mode!(Lyα, mode)
+
+0 (default): `adjoint_run!` differentiates each advection interface once in reverse mode;
+1: dual-number passes per cell, the cross-check of mode 0 (same gradient to rounding).
+"""
+set_adjoint_mode!(Lyα, mode::Integer) =
+    check(ccall((:m1lya_set_adjoint_mode, libm1lya), Cint, (Ptr{Cvoid}, Cint), ctxof(Lyα), mode), ctxof(Lyα))
 
 # --- device-side coefficient refresh (m1lya_set_voigt_lut, m1lya_prepare_subcycling) ---
 
