@@ -422,4 +422,87 @@ __device__ RowT<T> tfreq_row(const RowConst& c, T Hvm, T Hv0, T Hvp, T PREFACT, 
     return r;
 }
 
+// 1 - ϕ and χ of tomphi_L / tomphi_R with their derivatives along P (0 where P is clamped
+// to ±kPmax, where the forward's values are constants)
+__device__ __forceinline__ void omphi_d(bool left, double P, double& o, double& chi, double& do_, double& dchi)
+{
+    if (P >= kPmax || P <= -kPmax) {
+        const double e = left ? (P >= kPmax ? -kPmax : kPmax) : (P >= kPmax ? kPmax : -kPmax);
+        o = exp_det(e);
+        chi = (1.0 - o) / (P >= kPmax ? kPmax : -kPmax);
+        do_ = 0.0; dchi = 0.0;
+        return;
+    }
+    if (fabs(P) < kPmin) {
+        if (left) { o = 1.0 - P + 0.5 * (P * P); chi = 1.0 - 0.5 * P; do_ = -1.0 + P; }
+        else { o = 1.0 + P + 0.5 * (P * P); chi = -1.0 - 0.5 * P; do_ = 1.0 + P; }
+        dchi = -0.5;
+        return;
+    }
+    o = exp_det(left ? -P : P);
+    chi = (1.0 - o) / P;
+    do_ = left ? -o : o;
+    dchi = (-do_ - chi) / P;
+}
+
+// Reverse mode of tfreq_row: cotangents of (F_{k+1}, F_k, F_{k-1}) -> of its six inputs
+// g = (Hvoigt_{k-1}, Hvoigt_k, Hvoigt_{k+1}, PREFACT, RECOIL_0, VEL); the values follow the
+// forward's operations and branches, so the result equals the dual-number passes to rounding.
+__device__ inline void freq_row_vjp(const RowConst& c, double Hvm, double Hv0, double Hvp, double PREFACT,
+                                    double RECOIL_0, double VEL, double lFkp, double lFk, double lFkm, double g[6])
+{
+    const double Kkp = PREFACT * Hvp, Kk = PREFACT * Hv0, Kkm = PREFACT * Hvm;
+    double Ckp = VEL * c.nuCkp, Ck = VEL * c.nuCk, Ckm = VEL * c.nuCkm;
+    if (c.recoil) {
+        Ckp = Ckp + Kkp * (RECOIL_0 - c.two_inv_kp);
+        Ck = Ck + Kk * (RECOIL_0 - c.two_inv_k);
+        Ckm = Ckm + Kkm * (RECOIL_0 - c.two_inv_km);
+    }
+    const double qkp = c.dkp * (1.0 / Kkp), qk = c.dk * (1.0 / Kk), qkm = c.dkm * (1.0 / Kkm);
+    const double xkp = Ckp * qkp, xk = Ck * qk, xkm = Ckm * qkm;
+    const bool pkp = !(xkp > kPmax) && !(xkp < -kPmax), pk = !(xk > kPmax) && !(xk < -kPmax);
+    const bool pkm = !(xkm > kPmax) && !(xkm < -kPmax);
+    const double Pkp = (xkp > kPmax) ? kPmax : ((xkp < -kPmax) ? -kPmax : xkp);
+    const double Pk = (xk > kPmax) ? kPmax : ((xk < -kPmax) ? -kPmax : xk);
+    const double Pkm = (xkm > kPmax) ? kPmax : ((xkm < -kPmax) ? -kPmax : xkm);
+    double oRkp, xRkp, oLk, xLk, oRk, xRk, oLkm, xLkm;
+    double doRkp, dxRkp, doLk, dxLk, doRk, dxRk, doLkm, dxLkm;
+    omphi_d(false, Pkp, oRkp, xRkp, doRkp, dxRkp);
+    omphi_d(true, Pk, oLk, xLk, doLk, dxLk);
+    omphi_d(false, Pk, oRk, xRk, doRk, dxRk);
+    omphi_d(true, Pkm, oLkm, xLkm, doLkm, dxLkm);
+    const double D1 = xLk * qk - xRkp * qkp, D2 = xLkm * qkm - xRk * qk;
+    const double den_kp = c.inv_dnu_k * (1.0 / D1), den_km = c.inv_dnu_k * (1.0 / D2);
+    // F_{k+1} = oR'_{k+1} den_kp, F_{k-1} = oL_{k-1} den_km, F_k = oL_k den_kp + oR_k den_km
+    const double loRkp = lFkp * den_kp, loLkm = lFkm * den_km;
+    const double loLk = lFk * den_kp, loRk = lFk * den_km;
+    const double lden_kp = lFkp * oRkp + lFk * oLk, lden_km = lFkm * oLkm + lFk * oRk;
+    const double lD1 = -lden_kp * den_kp / D1, lD2 = -lden_km * den_km / D2;
+    const double lxLk = lD1 * qk, lxRkp = -lD1 * qkp, lxLkm = lD2 * qkm, lxRk = -lD2 * qk;
+    double lqk = lD1 * xLk - lD2 * xRk, lqkp = -lD1 * xRkp, lqkm = lD2 * xLkm;
+    const double lPkp = loRkp * doRkp + lxRkp * dxRkp;
+    const double lPk = loLk * doLk + lxLk * dxLk + loRk * doRk + lxRk * dxRk;
+    const double lPkm = loLkm * doLkm + lxLkm * dxLkm;
+    // P = clamp(C q)
+    const double lCkp = pkp ? lPkp * qkp : 0.0, lCk = pk ? lPk * qk : 0.0, lCkm = pkm ? lPkm * qkm : 0.0;
+    lqkp += pkp ? lPkp * Ckp : 0.0;
+    lqk += pk ? lPk * Ck : 0.0;
+    lqkm += pkm ? lPkm * Ckm : 0.0;
+    // q = dk / K, C = VEL νC (+ K (RECOIL_0 - 2/νC))
+    double lKkp = -lqkp * qkp / Kkp, lKk = -lqk * qk / Kk, lKkm = -lqkm * qkm / Kkm;
+    double lR0 = 0.0;
+    if (c.recoil) {
+        lKkp += lCkp * (RECOIL_0 - c.two_inv_kp);
+        lKk += lCk * (RECOIL_0 - c.two_inv_k);
+        lKkm += lCkm * (RECOIL_0 - c.two_inv_km);
+        lR0 = lCkp * Kkp + lCk * Kk + lCkm * Kkm;
+    }
+    g[0] = lKkm * PREFACT;
+    g[1] = lKk * PREFACT;
+    g[2] = lKkp * PREFACT;
+    g[3] = lKkp * Hvp + lKk * Hv0 + lKkm * Hvm;
+    g[4] = lR0;
+    g[5] = lCkp * c.nuCkp + lCk * c.nuCk + lCkm * c.nuCkm;
+}
+
 }  // namespace m1a

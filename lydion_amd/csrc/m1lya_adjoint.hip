@@ -648,8 +648,11 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
         Edd D = D_edd(X2r[k], X2z[k], X2J[k]);
         const double VEL = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr) * kInvC;
         const m1a::RowConst rc = row_const(k);
-        // derivatives along the six inputs, two per pass
+        // derivatives along the six inputs: reverse mode, or (mode 1) two per dual pass
         double g[6];
+        if (a.mode == 0) {
+            m1a::freq_row_vjp(rc, Hv[k - 1], Hv[k], Hv[k + 1], PREFACT, RECOIL_0, VEL, lFkp, lFk, lFkm, g);
+        } else {
 #pragma unroll 1
         for (int dp = 0; dp < 3; ++dp) {
             auto in = [&](double v, int d) { return Dl2(v, d == 2 * dp ? 1.0 : 0.0, d == 2 * dp + 1 ? 1.0 : 0.0); };
@@ -658,6 +661,7 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
             m1a::RowT<Dl2> row = m1a::tfreq_row<Dl2>(rc, hm, h0, hp, pf, rec, vel);
             g[2 * dp] = lFkp * row.Fkp.d0 + lFk * row.Fk.d0 + lFkm * row.Fkm.d0;
             g[2 * dp + 1] = lFkp * row.Fkp.d1 + lFk * row.Fk.d1 + lFkm * row.Fkm.d1;
+        }
         }
         if (!a.recoil) g[4] = 0.0;
         gm[k] = g[0];

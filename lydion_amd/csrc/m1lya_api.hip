@@ -1098,6 +1098,7 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
                 ca.g_aF = gbin; ca.g_Hv = gbin + n3; ca.g_phi = gbin + 2 * n3;
                 ca.g_alpha = gcell; ca.g_alpha_c = gcell + nc; ca.g_js = gcell + 2 * nc;
                 ca.g_dnuD = gcell + 3 * nc; ca.g_T = gcell + 4 * nc;
+                ca.mode = c->adj_mode;
                 {
                     Prof p(c, "adj_column");
                     launch_adj_column(st, ca);

@@ -201,6 +201,7 @@ struct AdjColArgs {
     double* g_js;
     double* g_dnuD;
     double* g_T;
+    int mode;             // 0 frequency rows in reverse mode; 1 dual-number passes
 };
 
 void launch_adj_column(hipStream_t s, const AdjColArgs& a);
