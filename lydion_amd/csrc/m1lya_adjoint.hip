@@ -472,6 +472,62 @@ __global__ __launch_bounds__(256) void k_adj_gather_rev(AdjAdvArgs a)
 // its transpose for mu) run on lane 0, interleaved.
 // ---------------------------------------------------------------------------
 namespace {
+// Tridiagonal solve of n <= 128 rows (row r: A[r] x[r-1] + B[r] x[r] + C[r] x[r+1] = d[r], an
+// M-matrix, no pivoting) by one wavefront: lane l holds rows 2l, 2l+1; one cyclic-reduction level
+// folds the odd rows into the even ones, parallel cyclic reduction over the 64 lanes solves the
+// even rows, back substitution gives the odd ones.  transposed: solve M^T instead (sub-diagonal
+// C[r-1], super-diagonal A[r+1]).  rhs(r) gives d[r]; x receives the solution (lane-own rows,
+// written after every read, so x may alias the storage rhs reads).
+template <class Rhs>
+__device__ __forceinline__ void tri_pcr(int n, const double* A, const double* B, const double* C, Rhs rhs,
+                                        bool transposed, double* x)
+{
+    const int lane = threadIdx.x & 63;
+    double ra[2], rb[2], rc[2], rd[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = 2 * lane + q;
+        if (r < n) {
+            ra[q] = (r == 0) ? 0.0 : (transposed ? C[r - 1] : A[r]);
+            rc[q] = (r == n - 1) ? 0.0 : (transposed ? A[r + 1] : C[r]);
+            rb[q] = B[r];
+            rd[q] = rhs(r);
+        } else {
+            ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = 0.0;
+        }
+    }
+    // odd row into the even rows: x1 = ud - ua x0 - uc x0(next lane)
+    const double ib1 = 1.0 / rb[1];
+    const double ua = ra[1] * ib1, uc = rc[1] * ib1, ud = rd[1] * ib1;
+    double pua = __shfl_up(ua, 1, 64), puc = __shfl_up(uc, 1, 64), pud = __shfl_up(ud, 1, 64);
+    if (lane == 0) { pua = 0.0; puc = 0.0; pud = 0.0; }
+    double b = rb[0] - ra[0] * puc - rc[0] * ua;
+    double d = rd[0] - ra[0] * pud - rc[0] * ud;
+    double a = -ra[0] * pua;
+    double c = -rc[0] * uc;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const double ib = 1.0 / b;
+        const double u = a * ib, v = c * ib, w = d * ib;
+        double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
+        double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
+        if (lane < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
+        if (lane + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
+        const double nb = b - a * vb - c * uf;
+        const double nd = d - a * wb - c * wf;
+        a = -a * ub;
+        c = -c * vf;
+        b = nb;
+        d = nd;
+    }
+    const double x0 = d / b;
+    double x0n = __shfl_down(x0, 1, 64);
+    if (lane == 63) x0n = 0.0;
+    const double x1 = ud - ua * x0 - uc * x0n;
+    if (2 * lane < n) x[2 * lane] = x0;
+    if (2 * lane + 1 < n) x[2 * lane + 1] = x1;
+}
+
 __device__ __forceinline__ double wsum(double v)
 {
 #pragma unroll
@@ -612,8 +668,12 @@ __global__ __launch_bounds__(64) void k_adj_column(AdjColArgs a)
     }
     __syncthreads();
 
-    // ---- the forward system M x = -dw J2 (Thomas; M is an M-matrix) and M^T mu = lJ3 ----
-    if (lane == 0 && n >= 1) {
+    // ---- the forward system M x = -dw J2 and M^T mu = lJ3 (M is an M-matrix): cyclic
+    // reduction over the wavefront's lanes up to 128 rows, else Thomas on lane 0 ----
+    if (n >= 1 && n <= 128) {
+        tri_pcr(n, rA, rB, rC, [&](int r) { return -dw * X2J[r + 1]; }, false, x);
+        tri_pcr(n, rA, rB, rC, [&](int r) { return mu[r]; }, true, mu);
+    } else if (lane == 0 && n >= 1) {
         double cpr = rC[0] / rB[0], xr = (-dw * X2J[1]) / rB[0];
         double cqr = (n > 1 ? rA[1] : 0.0) / rB[0], mur = mu[0] / rB[0];
         cp[0] = cpr; x[0] = xr; cq[0] = cqr; mu[0] = mur;

@@ -58,12 +58,12 @@ def _adjoint(cloud, n, mf, sp, checkpoint_every=0):
     return res, ctx
 
 
-def _test_cloud(lut, hll=False, bc_z_bot=L.REFLECT, T0=1e4, source="point", rest=False):
-    """12x10 cloud, 32 bins, τ0 = 1e6, dust, Hubble outflow of 200 km/s, destruction
+def _test_cloud(lut, hll=False, bc_z_bot=L.REFLECT, T0=1e4, source="point", rest=False, n_nu=32):
+    """12x10 cloud, 32 bins (or n_nu), τ0 = 1e6, dust, Hubble outflow of 200 km/s, destruction
     probability p_d up to 1e-3; a developed radiation field, or J = H = 0 (rest)."""
     rs = 0.5 * (1.1 / 12) if source == "point" else 1.0
     cloud = setup_static_cloud(1e6, T0, tau_abs=0.5, rs=rs, umax=200.0, n_r=12, n_z=10, bc_z_bot=bc_z_bot,
-                               nu=config_grid(32, 1e6, T_ref=T0), lut=lut)
+                               nu=config_grid(n_nu, 1e6, T_ref=T0), lut=lut)
     if not rest:
         cloud = developed_state(cloud)
     cloud.lya.hll = hll
@@ -129,6 +129,13 @@ def test_directional_derivatives_vs_finite_differences(lut, case):
         cloud = _test_cloud(lut, hll=True, bc_z_bot=L.OUTFLOW, T0=1e2, source="uniform")
         families = [f for f in FAMILIES if f[0] != "phi"]
     _directional_check(cloud, 5, families)
+
+
+def test_directional_derivatives_160_bins(lut):
+    """More than 130 bins: the reverse frequency solve runs on one lane (Thomas) instead of the
+    wavefront's cyclic reduction."""
+    cloud = _test_cloud(lut, hll=False, bc_z_bot=L.REFLECT, T0=1e4, source="point", n_nu=160)
+    _directional_check(cloud, 3, [("J", "state"), ("alpha_F", "bin"), ("Hvoigt", "bin"), ("alpha", "cell")])
 
 
 def test_emission_profile_gradient_from_rest(lut):
