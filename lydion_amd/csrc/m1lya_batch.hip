@@ -1884,8 +1884,8 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         b->profiling = 0;
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         set_ctrl(b, nullptr, 0);
-        // segment lengths: powers of two (8 gives 4 wavefronts per SIMD, the kernel's
-        // occupancy, on 128 rows of 256 cells) and the lengths that give 3 or 6 wavefronts
+        // segment lengths: powers of two (16 gives 4 wavefronts per SIMD, the kernel's
+        // occupancy, on 256 rows of 256 cells) and the lengths that give 3 or 6 wavefronts
         // per SIMD on 256-cell rows; 2, 4 or 8 wavefronts per block (taller blocks share
         // more Z interfaces and halo rows, shorter ones fit more blocks per CU)
         static const int seg_opts[] = {6, 8, 11, 16, 22, 32, 43, 64};
