@@ -118,11 +118,18 @@ def test_reductions_match(lut):
     assert rel(ng, no) < RED_TOL
 
 
-@pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow", "odd_grid_outflow"])
+@pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow", "odd_grid_outflow",
+                                  "reference_grid_194_bins"])
 def test_fused_step_bit_exact(lut, case):
     if case == "static_cloud_32x32x64":
         cloud = L.static_cloud_32x32x64(lut=lut)
         nsteps = 40
+    elif case == "reference_grid_194_bins":
+        # the reference's default frequency grid (q = 1.1, 194 bins; four bins per lane)
+        from lydion_amd.static_cloud import setup_static_cloud
+        cloud = randomize(setup_static_cloud(1e8, 1e4, tau_abs=0.5, rs=0.5 * (1.1 / 14), umax=50.0, n_r=14, n_z=12,
+                                             nu=L.reference_freq_grid(), lut=lut), seed=17)
+        nsteps = 4
     elif case == "odd_grid_outflow":
         # N_r and N_Z not multiples of the advection block (2 x 4 cells), open bottom
         cloud = randomize(small_cloud(n_r=13, n_z=11, n_nu=48, tau0=1e7, tau_abs=1.0, umax=100.0,
