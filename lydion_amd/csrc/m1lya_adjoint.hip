@@ -158,32 +158,54 @@ __device__ m1a::FaceT<T> face_at(const AdjAdvArgs& a, int dir, int f, int i, int
 // ---------------------------------------------------------------------------
 // Cell update of AdvectJandH_Cylindrical! (src/AdvectJandH.jl:1302-1382) in reverse.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_adj_cell(AdjAdvArgs a)
+// Block of 64 bins x BI x BJ cells, a wavefront per cell; the forward values of each interface
+// are evaluated once (as k_advect does) and reach the two cells that share it through LDS.
+template <int BI, int BJ>
+__global__ __launch_bounds__(64 * BI * BJ) void k_adj_cell(AdjAdvArgs a)
 {
-    const int nn = a.nn, nr = a.g.nr;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int i = blockIdx.y * 4 + threadIdx.y;
-    const int j = blockIdx.z;
-    if (k >= nn || i >= nr) return;
+    __shared__ double rF[BJ][BI + 1][5][64];   // F_J, F_Hr, F_HZ, K_ϕϕ left, K_ϕϕ right
+    __shared__ double zF[BJ + 1][BI][3][64];
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int lane = threadIdx.x, il = threadIdx.y, jl = threadIdx.z;
+    const int k = blockIdx.x * 64 + lane;
+    const int i = blockIdx.y * BI + il;
+    const int j = blockIdx.z * BJ + jl;
+    const bool cell_ok = k < nn && i < nr && j < nz;
+    const bool act = cell_ok && k >= a.krange[0] && k <= a.krange[1];
+    if (act) {
+        auto put_r = [&](int f, int slot) {
+            const m1a::FaceT<double> r = face_at<double>(a, 0, f, i, j, k, -1, -1);
+            double* o = &rF[jl][slot][0][lane];
+            o[0] = r.FJ; o[64] = r.FHr; o[128] = r.FHz; o[192] = r.kphi_L; o[256] = r.kphi_R;
+        };
+        auto put_z = [&](int g, int slot) {
+            const m1a::FaceT<double> r = face_at<double>(a, 1, g, i, j, k, -1, -1);
+            double* o = &zF[slot][il][0][lane];
+            o[0] = r.FJ; o[64] = r.FHr; o[128] = r.FHz;
+        };
+        put_r(i, il);
+        if (il == BI - 1 || i == nr - 1) put_r(i + 1, il + 1);
+        put_z(j, jl);
+        if (jl == BJ - 1 || j == nz - 1) put_z(j + 1, jl + 1);
+    }
+    __syncthreads();
+    if (!cell_ok) return;
     const size_t idx = ((size_t)j * nr + i) * nn + k;
     const double lJ3 = a.lJ1[idx], lHr3 = a.lHr1[idx], lHz3 = a.lHz1[idx];
-    if (k < a.krange[0] || k > a.krange[1]) {
+    if (!act) {
         a.pJ[idx] = 0.0; a.pHr[idx] = 0.0; a.pHz[idx] = 0.0;
         a.oJ[idx] = lJ3; a.oHr[idx] = lHr3; a.oHz[idx] = lHz3;
         return;
     }
-    // forward values of the four interfaces (as k_advect)
-    double Fr[2][3], Fz[2][3], kphi_im = 0.0, kphi_ip = 0.0;
-#pragma unroll 1
-    for (int side = 0; side < 4; ++side) {
-        const int dir = side < 2 ? 0 : 1;
-        const int f = dir == 0 ? i + side : j + side - 2;
-        m1a::FaceT<double> r = face_at<double>(a, dir, f, i, j, k, -1, -1);
-        double* F = dir == 0 ? Fr[side] : Fz[side - 2];
-        F[0] = r.FJ; F[1] = r.FHr; F[2] = r.FHz;
-        if (side == 0) kphi_im = r.kphi_R;
-        if (side == 1) kphi_ip = r.kphi_L;
+    double Fr[2][3], Fz[2][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        Fr[0][m] = rF[jl][il][m][lane];
+        Fr[1][m] = rF[jl][il + 1][m][lane];
+        Fz[0][m] = zF[jl][il][m][lane];
+        Fz[1][m] = zF[jl + 1][il][m][lane];
     }
+    const double kphi_im = rF[jl][il][4][lane], kphi_ip = rF[jl][il + 1][3][lane];
     const int c = j * nr + i;
     const double Ar_L = a.g.A_r[(size_t)j * (nr + 1) + i], Ar_R = a.g.A_r[(size_t)j * (nr + 1) + i + 1];
     const double Az_L = a.g.A_z[(size_t)j * nr + i], Az_R = a.g.A_z[(size_t)(j + 1) * nr + i];
@@ -852,7 +874,9 @@ void launch_adj_column(hipStream_t s, const AdjColArgs& a)
 void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a)
 {
     dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
-    hipLaunchKernelGGL(k_adj_cell, grid, dim3(64, 4), 0, s, a);
+    constexpr int BI = 2, BJ = 4;   // as k_advect: 2.75 interface evaluations per cell
+    hipLaunchKernelGGL((k_adj_cell<BI, BJ>), dim3((a.nn + 63) / 64, (a.g.nr + BI - 1) / BI, (a.g.nz + BJ - 1) / BJ),
+                       dim3(64, BI, BJ), 0, s, a);
     if (a.mode == 1) {
         hipLaunchKernelGGL(k_adj_gather, grid, dim3(64, 4), 0, s, a);
         return;
