@@ -340,16 +340,17 @@ int  m1lya_batch_kernel_stats(m1lya_batch *b, int index, char *name, size_t name
                               long *launches, double *total_ms);
 int  m1lya_batch_reset_kernel_stats(m1lya_batch *b);
 
-/* ---- instrumentation ------------------------------------------------- */
-/* When on, every kernel launch is bracketed by HIP events on the context's
- * stream; m1lya_kernel_stats() reports per-kernel launch counts and total
- * device milliseconds (synchronises). */
 /* How m1lya_adjoint_run differentiates the interfaces of the advection.  0 (default):
  * each interface once in reverse mode, its input cotangents gathered per cell.
  * 1: per cell, dual-number passes through the <= 8 interfaces whose stencil holds it
  * (16 passes; kept to cross-check mode 0).  The two agree to rounding.
  * No counterpart in the reference, which has no adjoint. */
 int m1lya_set_adjoint_mode(m1lya_ctx *ctx, int mode);
+
+/* ---- instrumentation ------------------------------------------------- */
+/* When on, every kernel launch is bracketed by HIP events on the context's
+ * stream; m1lya_kernel_stats() reports per-kernel launch counts and total
+ * device milliseconds (synchronises). */
 int m1lya_set_profiling(m1lya_ctx *ctx, int on);
 int m1lya_kernel_stats_count(m1lya_ctx *ctx);
 int m1lya_kernel_stats(m1lya_ctx *ctx, int index, char *name, size_t name_len,

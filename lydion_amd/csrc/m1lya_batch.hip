@@ -85,6 +85,18 @@ struct Cloud {
     int bc_r_right, bc_z_bot, bc_z_top, has_vel;
     float cdt, cdtc;               // c̃Δt and c̃Δt/c for the FP32 flux sweep
     double c_tilde, dt, z_center, J0, alpha_floor;
+    // derived on the host when the descriptors are uploaded (sync_clouds)
+    double cdt64;                  // c̃Δt
+    double dw;                     // 1/(c̃Δt), the diagonal of the backward-Euler rows
+    double src_scale;              // c̃Δt/J0: emission ∫js·ϕ_ν in units of J0
+    double mf_scale;               // (4π/c)(1/c̃)·J0/Δt: Σ Δν ΔH in units of J0 -> f_Lyα
+    // this cloud's slices of the batch arrays (set in sync_clouds)
+    float* sJ[2];
+    float* sHr[2];
+    float* sHz[2];
+    const float *saF, *sHv, *sphi, *sivr, *sivz, *sucr, *sucz;
+    const CellRec* srec;
+    double *sfr, *sfz;
 };
 
 struct Seg {
@@ -377,49 +389,44 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
     const bool row_valid = (j < nz);                    // rows above the cloud are ghost rows
     const int nvalid = min(wpb, nz - j0);
     const int hll = cl.hll, bcr = cl.bc_r_right, bcb = cl.bc_z_bot, bct = cl.bc_z_top;
-    const float* J = A.J[parity] + cl.o3;
-    const float* Hr = A.Hr[parity] + cl.o3;
-    const float* Hz = A.Hz[parity] + cl.o3;
-    float* Jw = A.J[parity ^ 1] + cl.o3;
-    float* Hrw = A.Hr[parity ^ 1] + cl.o3;
-    float* Hzw = A.Hz[parity ^ 1] + cl.o3;
-    const float* aF = A.aF + cl.o3;
-    const float* Hvp = A.Hv + cl.o3;
-    const float* php = A.phi + cl.o3;
-    const float* ivr = A.ivr + cl.oivr;
-    const float* ivz = A.ivz + cl.oivz;
-    const float* ucr = A.uc_r + cl.ofr;
-    const float* ucz = A.uc_z + cl.ofz;
-    const CellRec* rec = A.rec + cl.oc;
+    // The cloud's descriptor (step constants and array slices) is read where it is used,
+    // through the constant address space: the loads are scalar and invariant, so a field
+    // can be loaded again instead of being held across the cell loop in an SGPR (the kernel
+    // is at the SGPR limit, and spilled SGPRs cost VALU lane reads and writes).
+    using ConstCloud = __attribute__((address_space(4))) const Cloud;
+    auto cloud_now = [&]() -> ConstCloud& { return *(ConstCloud*)(clouds + cidx); };
     for (int t = threadIdx.x; t < nn; t += blockDim.x) sbin[t] = A.bin[cl.obin + t];
     __syncthreads();
     const BinRec* bins = sbin;
     const int kmin = krange[2 * cidx], kmax = krange[2 * cidx + 1];
 
-    LaneBins L;
+    // The lane's bins and their lane masks.  Inside the cell loop they are rebuilt from a
+    // lane index made opaque in every iteration: a compare costs one instruction there,
+    // while masks hoisted out of the loop occupy SGPR pairs that spill to VGPR lanes.
+    auto lane_bins = [&](int ln) {
+        LaneBins B;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        L.kb[q] = lane * R + q;
-        L.val[q] = (q < R) && (L.kb[q] < nn);
-        L.act[q] = L.val[q] && L.kb[q] >= kmin && L.kb[q] <= kmax;
-    }
+        for (int q = 0; q < 2; ++q) {
+            B.kb[q] = ln * R + q;
+            B.val[q] = (q < R) && (B.kb[q] < nn);
+            B.act[q] = B.val[q] && B.kb[q] >= kmin && B.kb[q] <= kmax;
+        }
+        return B;
+    };
+    LaneBins L = lane_bins(lane);
 
-    const double ct = cl.c_tilde, dt = cl.dt, cdt = ct * dt, dw = 1.0 / cdt;
-    const double inv_J0 = 1.0 / cl.J0;
-    const double fpc_ict = (kFourPi / kC) / ct * cl.J0;
-    const double inv_dt = 1.0 / dt;
-    const float cdtf = cl.cdt, cdtcf = cl.cdtc;
     const int i0 = sg.i0;
 
     // Ghost-aware (J, H_r/J, H_Z/J) of cell (i, row) for the Z exchange; rows beyond
     // N_Z + 1 are never part of a needed stencil and are clamped.
     auto load_zrow = [&](int i, int row, int slot) {
         row = min(row, nz + 1);
+        auto& cp = cloud_now();
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             if (!L.act[q]) continue;
             float vj, vr, vz, fr, fz;
-            ld_z(J, Hr, Hz, nn, nr, nz, i, L.kb[q], row, bcb, bct, vj, vr, vz);
+            ld_z(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, nz, i, L.kb[q], row, bcb, bct, vj, vr, vz);
             m1f::flux_ratios(vj, vr, vz, fr, fz);
             const int p = q * 64 + lane;
             zrow[slot][0][p] = vj;
@@ -439,8 +446,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             // cells i0-2, i0-1, i0 in slots 1..3: the first shift and load of the
             // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1)
             if (row_valid && L.val[q] && m >= 1) {
+                auto& cp = cloud_now();
                 float vr, vz;
-                ld_r(J, Hr, Hz, nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m], vr, vz);
+                ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m],
+                     vr, vz);
                 m1f::flux_ratios(wJ[q][m], vr, vz, wR[q][m], wZ[q][m]);
             }
         }
@@ -451,6 +460,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
     // All wavefronts of the block march over the same cells [i0, i1) in lockstep;
     // i = i0-1 only evaluates the left r interface of cell i0.
     for (int i = i0 - 1; i < sg.i1; ++i) {
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        L = lane_bins(ln);
         const bool first = (i < i0);
         const size_t cell = (size_t)j * nr + i;   // used only for valid rows, i >= i0
         const size_t base = cell * nn;
@@ -458,6 +470,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         float af_c[2] = {0.f, 0.f};   // α_F of cell i (the left side of its right r interface)
         // ---- right r interface of cell i (FP32) ----
         float FR[2][3], kip[2], kimN[2];
+        {
+        auto& cp = cloud_now();
+        const float* aF = cp.saF;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             FR[q][0] = FR[q][1] = FR[q][2] = 0.f;
@@ -469,7 +484,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
             {
                 float vr, vz;
-                ld_r(J, Hr, Hz, nn, nr, j, k, i + 2, bcr, wJ[q][3], vr, vz);
+                ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, k, i + 2, bcr, wJ[q][3], vr, vz);
                 m1f::flux_ratios(wJ[q][3], vr, vz, wR[q][3], wZ[q][3]);
             }
             af_c[q] = aF[((size_t)j * nr + max(i, 0)) * nn + k];
@@ -480,11 +495,13 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             st.fz[0] = wZ[q][1]; st.fz[1] = wZ[q][2]; st.fz[2] = wZ[q][3]; st.fz[3] = wZ[q][0];
             const float aL = af_c[q];
             const float aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
-            const float u = ucr[(size_t)j * (nr + 1) + i + 1];
-            const m1f::Face32 f = m1f::interface_flux(0, hll, st, ivr + (size_t)(i + 1) * m1f::kIvStride, aL, aR, u);
+            const float u = cp.sucr[(size_t)j * (nr + 1) + i + 1];
+            const m1f::Face32 f =
+                m1f::interface_flux(0, hll, st, cp.sivr + (size_t)(i + 1) * m1f::kIvStride, aL, aR, u);
             FR[q][0] = f.FJ; FR[q][1] = f.FHr; FR[q][2] = f.FHz;
             kip[q] = f.kphi_L;
             kimN[q] = (i + 1 == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
+        }
         }
         if (first) {
 #pragma unroll
@@ -515,6 +532,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         __syncthreads();
         for (int f = wave; f <= nvalid; f += wpb) {
             const int g = j0 + f;
+            auto& cp = cloud_now();
+            const float* aF = cp.saF;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 if (!L.act[q]) continue;
@@ -530,8 +549,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 }
                 const float aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
                 const float aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
-                const m1f::Face32 r = m1f::interface_flux(1, hll, st, ivz + (size_t)g * m1f::kIvStride, aL, aR,
-                                                          ucz[(size_t)g * nr + i]);
+                const m1f::Face32 r = m1f::interface_flux(1, hll, st, cp.sivz + (size_t)g * m1f::kIvStride, aL, aR,
+                                                          cp.sucz[(size_t)g * nr + i]);
                 zflx[f][0][p] = r.FJ;
                 zflx[f][1][p] = r.FHr;
                 zflx[f][2][p] = r.FHz;
@@ -540,7 +559,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         __syncthreads();
         if (!row_valid) continue;
 
-        const CellRec cr = rec[cell];
+        auto& cv = cloud_now();
+        const CellRec cr = cv.srec[cell];
         double jo[2], hr[2], hz[2];
         // ---- cell update (FP32): src/AdvectJandH.jl:1319-1379 ----
 #pragma unroll
@@ -549,7 +569,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
             if (!L.val[q]) continue;
             const int k = L.kb[q];
-            const float Js = wJ[q][1], Hrs = Hr[base + k], Hzs = Hz[base + k];
+            const float Js = wJ[q][1], Hrs = cv.sHr[parity][base + k], Hzs = cv.sHz[parity][base + k];
             float Jn = Js, Hrn = Hrs, Hzn = Hzs;
             if (L.act[q]) {
                 const int p = q * 64 + lane;
@@ -557,20 +577,20 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 const float dJ = gZr * zflx[wave + 1][0][p] - gZl * zflx[wave][0][p];
                 const float dHr = gZr * zflx[wave + 1][1][p] - gZl * zflx[wave][1][p];
                 const float dHz = gZr * zflx[wave + 1][2][p] - gZl * zflx[wave][2][p];
-                Jn = Js - cdtf * ((gRr * FR[q][0] - gRl * FL[q][0]) + dJ);
+                Jn = Js - cv.cdt * ((gRr * FR[q][0] - gRl * FL[q][0]) + dJ);
                 Jn = m1f::fmaxf_(Jn, m1f::kJeps32);
-                Hzn = Hzs - cdtf * ((gRr * FR[q][2] - gRl * FL[q][2]) + dHz);
-                Hrn = Hrs - cdtf * ((gRr * FR[q][1] - gRl * FL[q][1]) + dHr);
-                Hrn += cdtf * (0.5f * (kimR[q] + kip[q])) * cr.inv_r;
-                if (cl.has_vel) {
+                Hzn = Hzs - cv.cdt * ((gRr * FR[q][2] - gRl * FL[q][2]) + dHz);
+                Hrn = Hrs - cv.cdt * ((gRr * FR[q][1] - gRl * FL[q][1]) + dHr);
+                Hrn += cv.cdt * (0.5f * (kimR[q] + kip[q])) * cr.inv_r;
+                if (cv.has_vel) {
                     const float durdr = (float)cr.durdr, durdz = (float)cr.durdz;
                     const float duzdr = (float)cr.duzdr, duzdz = (float)cr.duzdz, urr = (float)cr.urr;
                     const m1f::Edd32 D = m1f::D_edd(Hrs, Hzs, Js);
                     const float Dpp = m1f::fmaxf_(1.0f - D.rr - D.zz, 0.0f);
                     const float Ddu = D.rr * durdr + D.rz * (durdz + duzdr) + D.zz * duzdz + Dpp * urr;
-                    Hrn += -cdtcf * (Hrs * durdr + Hzs * durdz);
-                    Hzn += -cdtcf * (Hrs * duzdr + Hzs * duzdz);
-                    Jn = m1f::fmaxf_(Jn - cdtcf * Ddu * Js, m1f::kJeps32);
+                    Hrn += -cv.cdtc * (Hrs * durdr + Hzs * durdz);
+                    Hzn += -cv.cdtc * (Hrs * duzdr + Hzs * duzdz);
+                    Jn = m1f::fmaxf_(Jn - cv.cdtc * Ddu * Js, m1f::kJeps32);
                 }
                 // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
                 m1f::causal_limit(Jn, Hrn, Hzn);
@@ -583,7 +603,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         // ================= emission / absorption (FP64) =================
         // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462), J in units of J0;
         // the H deposit hr - hr/(1+x) is formed as hr·x/(1+x) (no cancellation)
-        const double al_pd = cr.alpha * cr.pd, src = cr.js * ct * dt * inv_J0;
+        const double al_pd = cr.alpha * cr.pd, src = cr.js * cv.src_scale;
+        const float* Hvp = cv.sHv;
+        const float* php = cv.sphi;
         double hv[2] = {1.0, 1.0};
         double sr = 0.0, sz = 0.0;
 #pragma unroll
@@ -593,8 +615,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             const double h = Hvp[idx];
             hv[q] = h;
             const double aJ = cr.alpha_c + al_pd * h;
-            const double jn = (jo[q] + src * (double)php[idx]) * drcp1(1.0 + aJ * cdt);
-            const double xH = (double)af_c[q] * cdt;
+            const double jn = (jo[q] + src * (double)php[idx]) * drcp1(1.0 + aJ * cv.cdt64);
+            const double xH = (double)af_c[q] * cv.cdt64;
             const double gH = xH * drcp1(1.0 + xH);          // 1 - 1/(1+x)
             const double dk = bins[L.kb[q]].dnu;
             sr += dk * (hr[q] * gH);
@@ -607,14 +629,14 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         sz = wave_sum64(sz);
         {
             // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
-            const double fr = fpc_ict * sr * inv_dt, fz = fpc_ict * sz * inv_dt;
+            const double fr = sr * cv.mf_scale, fz = sz * cv.mf_scale;
             if (lane == 0) {
-                A.f_r[cl.oc + cell] = fr;
-                A.f_z[cl.oc + cell] = fz;
+                cloud_now().sfr[cell] = fr;
+                cloud_now().sfz[cell] = fz;
             }
             kadd(S_MF, cr.dV * sqrt(fr * fr + fz * fz));
             kadd(S_L, kFourPi * cr.dV * cr.js);
-            const double ddr = cr.rcom, ddz = cr.zcom - cl.z_center;
+            const double ddr = cr.rcom, ddz = cr.zcom - cv.z_center;
             const double d2 = ddr * ddr + ddz * ddz;
             double frad = 0.0;
             if (d2 > 0.0) frad = (fr * ddr + fz * ddz) * drcp1(sqrt(d2));
@@ -633,7 +655,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         // G_{k-1/2}/Δν_k, Fk = (oL_k·G_{k+1/2} + oR'_k·G_{k-1/2})/Δν_k.  (The reference
         // row k+1 uses P_{k+1} where this uses P'_{k+1}; the two differ by the rounding
         // of the bin centre, ~1e-12 relative.)
-        const bool vel = cl.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
+        const bool vel = cv.has_vel && (cr.durdr != 0.0 || cr.durdz != 0.0 || cr.duzdr != 0.0 ||
                                         cr.duzdz != 0.0 || cr.urr != 0.0);
         // With an isotropic velocity gradient (∂u_r/∂r = ∂u_Z/∂Z = u_r/r, no shear) D:∇u is the
         // trace times the rate for every bin, so VEL is the same in all bins and the shared-G
@@ -642,8 +664,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         // its three neighbours (src/FreqFactors.jl:226-254).
         const bool aniso =
             kShear && vel && !((cr.durdz + cr.duzdr) == 0.0 && cr.durdr == cr.duzdz && cr.duzdz == cr.urr);
-        const double PREF = 0.5 * (cr.alpha > cl.alpha_floor ? cr.alpha : cl.alpha_floor) * (cr.dnuD * cr.dnuD);
-        const double R0 = cl.recoil ? kHoverKB / cr.T : 0.0;
+        const double PREF = 0.5 * (cr.alpha > cv.alpha_floor ? cr.alpha : cv.alpha_floor) * (cr.dnuD * cr.dnuD);
+        const double R0 = cv.recoil ? kHoverKB / cr.T : 0.0;
         const int R1 = R - 1;
         double ra[2], rb[2], rc[2], rd[2];
         if (!aniso) {
@@ -664,7 +686,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 const double K = PREF * hv[q];
                 const double iK = drcp1(K);
                 double C = VEL * bk.nuC;
-                if (cl.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
+                if (cv.recoil) C += K * (R0 - 2.0 * bk.inv_nuC);
                 const double qR = bk.half_R * iK, qL = bk.half_L * iK;
                 const double P = fmin(fmax(C * qR, -kPmax), kPmax);
                 const double Pp = fmin(fmax(C * qL, -kPmax), kPmax);
@@ -716,8 +738,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 // the couplings to bins 0 and N_ν-1, which keep their value
                 ra[q] = (k == 1) ? 0.0 : Fkm;
                 rc[q] = (k == nn - 2) ? 0.0 : Fkp;
-                rb[q] = -(Fk + dw);
-                rd[q] = -dw * jo[q];
+                rb[q] = -(Fk + cv.dw);
+                rd[q] = -cv.dw * jo[q];
             }
         } else {
             // one bin at a time (a scheduling barrier between them keeps the two rows' temporaries
@@ -728,7 +750,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 const int k = L.kb[q];
                 if (L.val[q] && k >= 1 && k <= nn - 2)
                     aniso_row(bins, Hvp + base, k, nn, hr[q], hz[q], jo[q], cr.durdr, cr.durdz, cr.duzdr,
-                              cr.duzdz, cr.urr, PREF, R0, cl.recoil, dw, ra[q], rb[q], rc[q], rd[q]);
+                              cr.duzdz, cr.urr, PREF, R0, cv.recoil, cv.dw, ra[q], rb[q], rc[q], rd[q]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -766,8 +788,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             const double u = a * ib, v = c * ib, w = d * ib;
             double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
             double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
-            if (lane < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
-            if (lane + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
+            if (ln < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
+            if (ln + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
             const double nb = b - a * vb - c * uf;
             const double nd = d - a * wb - c * wf;
             a = -a * ub;
@@ -803,9 +825,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             }
             const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
             const size_t idx = base + k;
-            Jw[idx] = Jf;
-            Hrw[idx] = hrf;
-            Hzw[idx] = hzf;
+            auto& cw = cloud_now();
+            cw.sJ[parity ^ 1][idx] = Jf;
+            cw.sHr[parity ^ 1][idx] = hrf;
+            cw.sHz[parity ^ 1][idx] = hzf;
             const double Jd = Jf, dk = bins[k].dnu;
             sh[wave][k] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
             sh[wave][nn + k] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
@@ -815,7 +838,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                         (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
                          Azb * fmax(-(double)hzf, 0.0));
         }
-        if (edge) kadd(S_ESC, wave_sum64(escl) * cl.J0);
+        if (edge) kadd(S_ESC, wave_sum64(escl) * cv.J0);
     }
 
     // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
@@ -1172,6 +1195,27 @@ void build_tiles(m1lya_batch* b)
 void sync_clouds(m1lya_batch* b)
 {
     if (!b->cl_dirty) return;
+    for (Cloud& cl : b->cl) {
+        cl.cdt64 = cl.c_tilde * cl.dt;
+        cl.dw = 1.0 / cl.cdt64;
+        cl.src_scale = cl.cdt64 / cl.J0;
+        cl.mf_scale = (kFourPi / kC) / cl.c_tilde * cl.J0 / cl.dt;
+        for (int p = 0; p < 2; ++p) {
+            cl.sJ[p] = b->A.J[p] + cl.o3;
+            cl.sHr[p] = b->A.Hr[p] + cl.o3;
+            cl.sHz[p] = b->A.Hz[p] + cl.o3;
+        }
+        cl.saF = b->A.aF + cl.o3;
+        cl.sHv = b->A.Hv + cl.o3;
+        cl.sphi = b->A.phi + cl.o3;
+        cl.sivr = b->A.ivr + cl.oivr;
+        cl.sivz = b->A.ivz + cl.oivz;
+        cl.sucr = b->A.uc_r + cl.ofr;
+        cl.sucz = b->A.uc_z + cl.ofz;
+        cl.srec = b->A.rec + cl.oc;
+        cl.sfr = b->A.f_r + cl.oc;
+        cl.sfz = b->A.f_z + cl.oc;
+    }
     bchk(b, hipMemcpy(b->d_cl, b->cl.data(), b->n * sizeof(Cloud), hipMemcpyHostToDevice), "cloud descriptors");
     b->cl_dirty = false;
 }
