@@ -530,13 +530,21 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             load_zrow(i, j0 - 2 + slot, slot);
         }
         __syncthreads();
-        for (int f = wave; f <= nvalid; f += wpb) {
+        // Wavefront w evaluates interface w.  The block's top interface (f = wpb) is split
+        // between wavefronts 0 and 1 by bin parity when every lane holds two bins, so that no
+        // wavefront evaluates two full interfaces before the barrier.
+        const bool split_top = (R == 2) && (wpb >= 2);
+#pragma unroll 1
+        for (int it = 0; it < 2; ++it) {
+            const int f = it ? wpb : wave;
+            const int qm = !it ? 3 : (wave == 0) ? (split_top ? 1 : 3) : (wave == 1 && split_top) ? 2 : 0;
+            if (f > nvalid || qm == 0) continue;
             const int g = j0 + f;
             auto& cp = cloud_now();
             const float* aF = cp.saF;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                if (!L.act[q]) continue;
+                if (!L.act[q] || !((qm >> q) & 1)) continue;
                 const int k = L.kb[q];
                 const int p = q * 64 + lane;
                 m1f::Stencil32 st;
