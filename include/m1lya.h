@@ -325,8 +325,8 @@ int  m1lya_batch_active_bins(m1lya_batch *b, int *out);
  * The state after a step does not depend on the tile shape. */
 int  m1lya_batch_set_tiles(m1lya_batch *b, int seg_len, int waves_per_block);
 int  m1lya_batch_get_tiles(const m1lya_batch *b, int *seg_len, int *waves_per_block);
-/* Times each candidate tile shape over trial_steps steps and keeps the fastest;
- * the state is restored afterwards.  Outputs may be NULL. */
+/* Times each candidate tile shape over trial_steps steps, in two passes over the
+ * candidates, and keeps the fastest; the state is restored afterwards.  Outputs may be NULL. */
 int  m1lya_batch_autotune(m1lya_batch *b, int trial_steps, int *seg_len, int *waves_per_block, double *best_ms);
 /* on != 0: steps replay a captured hipGraph (one per state-buffer parity). */
 int  m1lya_batch_set_graph(m1lya_batch *b, int on);

@@ -1947,6 +1947,9 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         hipEvent_t e0, e1;
         bchk(b, hipEventCreate(&e0), "hipEventCreate");
         bchk(b, hipEventCreate(&e1), "hipEventCreate");
+        // two passes over the candidates, each candidate keeping its faster pass: the first
+        // candidates of a single pass ran while the clocks were still rising
+        for (int pass = 0; pass < 2; ++pass)
         for (int s : seg_opts)
             for (int w : wpb_opts) {
                 b->seg_len = s;
