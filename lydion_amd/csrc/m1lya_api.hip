@@ -335,12 +335,19 @@ void check_err_flag(m1lya_ctx* c, const char* what)
     }
 }
 
+// tape_X1 (may be null): receives the advection output (J, H_r, H_Z), as replay_step records it
 void fused_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc_r_right, int bc_z_bot,
-                int bc_z_top, double* history_row)
+                int bc_z_top, double* history_row, double* tape_X1 = nullptr)
 {
     if (!c->krange_valid) compute_krange(c);
     const double cdt = c_tilde * dt;
     run_advect(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    if (tape_X1) {
+        const size_t b = c->n3 * sizeof(double);
+        hipchk(c, hipMemcpyAsync(tape_X1, c->J, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+        hipchk(c, hipMemcpyAsync(tape_X1 + c->n3, c->Hr, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+        hipchk(c, hipMemcpyAsync(tape_X1 + 2 * c->n3, c->Hz, b, hipMemcpyDeviceToDevice, c->stream), "tape");
+    }
     ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED);
     a.c_tilde = c_tilde; a.dt = dt; a.dw = 1.0 / cdt; a.z_center = z_center; a.bc_z_bot = bc_z_bot;
     Prof p(c, "column");
@@ -1025,12 +1032,19 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
         zero(c->drhou_r, nc);
         zero(c->drhou_z, nc);
 
-        // ---- forward sweep: checkpoints, active ranges, scalars ----
+        // ---- forward sweep: checkpoints, active ranges, scalars; the last segment's tape ----
         if (!c->krange_valid) compute_krange(c);
+        const int s_last = (nck - 1) * S;   // first step of the last segment, which is not replayed
         for (int s = 0; s < n_steps; ++s) {
             if (s % S == 0) copy3_from_state(ck + (size_t)(s / S) * B3);
             hipchk(c, hipMemcpyAsync(kr + 2 * s, c->krange, 2 * sizeof(int), hipMemcpyDeviceToDevice, st), "krange");
-            fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, hist + (size_t)8 * s);
+            double* tX1 = nullptr;
+            if (s >= s_last) {
+                double* Xn = tape + (size_t)2 * (s - s_last) * B3;
+                copy3_from_state(Xn);
+                tX1 = Xn + B3;
+            }
+            fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, hist + (size_t)8 * s, tX1);
         }
         copy3_from_state(fin);
         hipchk(c, hipMemcpyAsync(kr + 2 * n_steps, c->krange, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
@@ -1064,13 +1078,15 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
         // ---- reverse sweep, segment by segment ----
         for (int m = nck - 1; m >= 0; --m) {
             const int s0 = m * S, s1 = std::min(s0 + S, n_steps);
-            copy3_to_state(ck + (size_t)m * B3);
-            for (int s = s0; s < s1; ++s) {
-                double* Xn = tape + (size_t)2 * (s - s0) * B3;
-                copy3_from_state(Xn);
-                hipchk(c, hipMemcpyAsync(c->krange, kr + 2 * s, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
-                       "krange");
-                replay_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, Xn + B3);
+            if (m != nck - 1) {   // the last segment was taped by the forward sweep
+                copy3_to_state(ck + (size_t)m * B3);
+                for (int s = s0; s < s1; ++s) {
+                    double* Xn = tape + (size_t)2 * (s - s0) * B3;
+                    copy3_from_state(Xn);
+                    hipchk(c, hipMemcpyAsync(c->krange, kr + 2 * s, 2 * sizeof(int), hipMemcpyDeviceToDevice, st),
+                           "krange");
+                    replay_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, Xn + B3);
+                }
             }
             for (int s = s1 - 1; s >= s0; --s) {
                 const double* Xn = tape + (size_t)2 * (s - s0) * B3;
