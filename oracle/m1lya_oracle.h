@@ -1,8 +1,8 @@
 /*
  * m1lya_oracle.h -- CPU FP64 restatement of LYDION's multi-frequency M1 Lya step.
  *
- * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP library
- * (lydion_amd/libm1lya.so): only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY.  The HIP library (lydion_amd/libm1lya.so) is compared
+ * with this oracle: only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  Nothing in the product path may
  * link, import or call it.
  *

@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU FP64 oracle (oracle/liboracle_m1lya.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path never uses it.
+cpu_baseline leg of bench.py, as the reference the GPU results are compared
+with.  The product path never uses it.
 """
 from __future__ import annotations
 
