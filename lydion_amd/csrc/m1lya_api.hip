@@ -483,7 +483,7 @@ int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
         c->inv_dnu = dalloc(c, n_nu); c->inv_nuC = dalloc(c, n_nu);
         const int W = column_width();
         const int ngroups = (int)((nc + W - 1) / W);
-        c->col_blocks = std::min(ngroups, 2560);
+        c->col_blocks = column_blocks(n_nu, ngroups);
         c->act_blocks = (int)std::min<size_t>(nc, 1024);
         c->pstride = 2 * n_nu + 8;
         c->prows = std::max(c->col_blocks, c->act_blocks);

@@ -219,6 +219,9 @@ void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride
 void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange);
 void launch_advect(hipStream_t s, const AdvectArgs& a);
 int column_width();
+// Blocks of the column kernel for ngroups column groups of an N_ν-bin grid: as many as the
+// GPU holds at once (one launch wave), at most ngroups.
+int column_blocks(int nn, int ngroups);
 void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks);
 void launch_step_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int bc_z_bot,
                        int* krange, double* scalars, double* history);
