@@ -205,6 +205,12 @@ static void flux_hll(const double in[16], double l1L, double l1R, double l3L, do
     out[2] = eps * FHZ + half_ome * (FHZ_L + FHZ_R);
 }
 
+/* GetFluxHLL with the wave speeds as inputs: in as orc_flux_glf, l = {λ1_L, λ1_R, λ3_L, λ3_R}. */
+void orc_flux_hll(const double in[16], const double l[4], double out[3])
+{
+    flux_hll(in, l[0], l[1], l[2], l[3], out);
+}
+
 /* ReconstructInterface: src/AdvectJandH.jl:359-473.
  * in  = {J_i,J_ip1,J_ip2,J_im1, Hr_i,Hr_ip1,Hr_ip2,Hr_im1, HZ_i,HZ_ip1,HZ_ip2,HZ_im1,
  *        Δr_L,Δr_R, inv_i_im1, inv_ip1_i, inv_ip2_ip1, αF_i, αF_ip1}

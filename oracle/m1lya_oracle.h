@@ -80,6 +80,7 @@ void   orc_D_edd(double H_r, double H_z, double J, double out[3]);
 void   orc_D_edd_from_f_n(double f, double nr, double nz, double out[3]);
 void   orc_hll_eigenvalues(double f, double mu, double out[2]);
 void   orc_flux_glf(const double in[16], double out[3]);
+void   orc_flux_hll(const double in[16], const double l[4], double out[3]);
 void   orc_reconstruct_interface(const double in[19], double out[14]);
 double orc_Kphiphi_inner_axis(double J_axis, double HZ_axis);
 

@@ -70,6 +70,7 @@ def _declare(lib):
         "orc_exp": (d, [d]), "orc_minmod": (d, [d, d]), "orc_chi_L84": (d, [d]),
         "orc_D_edd": (None, [d, d, d, _dp]), "orc_D_edd_from_f_n": (None, [d, d, d, _dp]),
         "orc_hll_eigenvalues": (None, [d, d, _dp]), "orc_flux_glf": (None, [_dp, _dp]),
+        "orc_flux_hll": (None, [_dp, _dp, _dp]),
         "orc_reconstruct_interface": (None, [_dp, _dp]), "orc_Kphiphi_inner_axis": (d, [d, d]),
         "orc_gtsv": (i, [i, _dp, _dp, _dp, _dp]), "orc_thomas": (None, [i, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_find_active_bins": (None, [L, G, C.POINTER(i), C.POINTER(i)]),
