@@ -42,7 +42,8 @@ enum m1lya_status {
     M1LYA_ERR_ARG = -1,        /* bad argument (NULL pointer, wrong count, bad enum) */
     M1LYA_ERR_DIMS = -2,       /* unsupported dimensions (N_r < 3, N_Z < 3, N_ν < 3) */
     M1LYA_ERR_HIP = -3,        /* HIP runtime error (allocation, launch)            */
-    M1LYA_ERR_NONFINITE = -4,  /* non-finite value detected in the state             */
+    M1LYA_ERR_NONFINITE = -4,  /* a fused step wrote a non-finite J, H_r or H_Z, or its
+                                  tridiagonal system was singular                      */
     M1LYA_ERR_STATE = -5,      /* geometry / frequency grid not set                  */
     M1LYA_ERR_UNSUPPORTED = -6 /* option not implemented                             */
 };
@@ -210,7 +211,12 @@ int m1lya_edge_spectra(m1lya_ctx *ctx, double *raw, double *norm, size_t count);
  * fused into device kernels with no host round trip.  The velocity, T and
  * coefficient fields are the device-resident ones.  BC_r_left is the axis
  * (always reflecting).  out (may be NULL) = {M_F, M_F_radial_net,
- * M_F_radial_out, L_Lyα, f_esc, kmin, kmax, 0}; out != NULL synchronises. */
+ * M_F_radial_out, L_Lyα, f_esc, kmin, kmax, 0}; out != NULL synchronises.
+ * Every fused step checks that the J, H_r and H_Z it writes are finite.  A step
+ * queued without synchronisation (out == NULL) reports M1LYA_ERR_NONFINITE at the
+ * next call that synchronises the context: m1lya_step with out, m1lya_run_steps,
+ * m1lya_run_until_steady, m1lya_synchronize or m1lya_download (which still fills
+ * its host buffer). */
 int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
                int bc_r_right, int bc_z_bot, int bc_z_top, double out[8]);
 
@@ -218,6 +224,18 @@ int m1lya_step(m1lya_ctx *ctx, double c_tilde, double dt, double z_center,
  * receives n_steps rows of the 8 step scalars. */
 int m1lya_run_steps(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
                     int bc_r_right, int bc_z_bot, int bc_z_top, double *history);
+
+/* Fused steps until the state is steady: stops after the first step whose
+ *   ΔJ/J = Σ Δν ΔV |J_new - J_start| / Σ Δν ΔV max(J_new - Jeps, 0)
+ * is below tol, or after max_steps.  The reference runs a fixed horizon
+ * (src/StaticCloudTest.jl:317-322,357) and has no such stop; this is the
+ * "iterated to steady state (ΔJ/J < 1e-8)" option of the RT-only static cloud.
+ * history (may be NULL) receives *steps_done rows of the step scalars, whose
+ * last entry is that step's ΔJ/J.  The stop is decided on the device: steps
+ * are queued in chunks and those after the stop return without work. */
+int m1lya_run_until_steady(m1lya_ctx *ctx, int max_steps, double tol, double c_tilde, double dt,
+                           double z_center, int bc_r_right, int bc_z_bot, int bc_z_top,
+                           double *history, int *steps_done);
 
 /* ---- device-side coefficient refresh -------------------------------------- */
 typedef struct {
@@ -243,6 +261,54 @@ int m1lya_set_voigt_lut(m1lya_ctx *ctx, const double *luts, int nx, double xmax,
  * all bins (the RHD form), 0 over the core bins (Update_ϕν!, src/ConstantsAndFunctions.jl:362-393). */
 int m1lya_prepare_subcycling(m1lya_ctx *ctx, const double *T, const double *n_HI, const double *alpha_c,
                              const double *alpha_c_F, const double *js, const double *pd, int phi_all_bins);
+
+/* ---- RHD time-step control and the Lyα absorption sub-step ---------------- */
+/* GetRHDCFLTimeStep(r, Z, c̃, u_r, u_Z, CFL), src/AdvectJandH.jl:29-68: dt = CFL min over cells
+ * of 0.5 min(Δr, ΔZ) / (c̃ + |u|), a device reduction.  u_r, u_z [N_r, N_Z] cell velocities;
+ * the cell interfaces r, Z come from m1lya_set_geometry. */
+int m1lya_cfl_time_step(m1lya_ctx *ctx, double c_tilde, const double *u_r, const double *u_z, double cfl,
+                        double *dt);
+
+/* Inputs of FindRSL_LyαRT: hydro, chemistry and dust state, [N_r, N_Z] each (rho_d
+ * [3, N_r, N_Z]: silicate, graphite, PAH dust densities, species fastest, as Dust ρd). */
+typedef struct {
+    const double *u_r, *u_z, *gamma_ad, *P, *rho, *rho_d, *xHII, *xH2, *T;
+    double kappa_abs[3];       /* κd_abs_Lyα of silicate, graphite, PAH dust (DustState) */
+    m1lya_line_constants lc;   /* b(T), a(T) and σ0(T) prefactors */
+} m1lya_rsl_inputs;
+
+/* FindRSL_LyαRT(c̃_Lyα, c̃_min, Lyα, Hydro, Chem, Dust, Geom),
+ * src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:616-722: the largest per-cell RSL
+ * floor ϵ_RSL max(1, (aτ)^{1/3} f_abs) max(|u|, c_s) as a device reduction, clamped to
+ * [c̃_min, c] and to [0.75, 4] times c_old.  p_d is the device-resident field. */
+int m1lya_find_rsl(m1lya_ctx *ctx, double c_old, double c_min, const m1lya_rsl_inputs *in, double *c_new);
+
+/* The Lyα block of AbsorptionStep, src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:1577-1670,
+ * for every cell at once: from J_old, H_r_old, H_Z_old (device fields) over each cell's
+ * sub-step (dt_cell [N_r, N_Z], or dt for every cell when dt_cell is NULL) it writes
+ * J = max(backward Euler, Jeps), H_r, H_Z, and splits the momentum each bin deposits between
+ * gas and the dust species by flux opacity (SplitRadPressure, :1017-1048; rho_d [3, N_r, N_Z]
+ * may be NULL, kappa_F = the three κd_F_Lyα).  A cell whose e_Lyα (device field M1LYA_E_LYA,
+ * before the sub-step) changes by more than tol_Abs = 0.1 while dt > 0.1 / (c̃ (αc + p_d α)), or
+ * not finitely, is rejected (RejectAbsorption, :996) and keeps its old J and H
+ * (RestoreLyαRejection, :980-990).  Outputs (each may be NULL): redo [N_r, N_Z] (1 = rejected);
+ * cell_out [12][N_r, N_Z] = {Δe_Lyα, e_Lyα_new, Δρu_r_Lyα, Δρu_Z_Lyα, then r and Z of the
+ * gas, silicate, graphite and PAH shares}, zero but Δe_Lyα on a rejected cell.  e_Lyα itself is
+ * left to the caller, as in the reference (stored once the whole sub-step is accepted). */
+int m1lya_absorption_substep(m1lya_ctx *ctx, const double *dt_cell, double dt, double c_tilde,
+                             const double *rho_d, const double *kappa_F, int *redo, double *cell_out);
+
+/* PrepareLyαSubCycling! (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2086-2155) for every cell
+ * from the chemical state, all on the device: ∫dν js from recombinations (∫dν_js_recLyα,
+ * src/LymanAlphaEmission.jl:105-124; Case B and the on-the-spot Lyα probability when ots != 0)
+ * and collisional excitation (∫dν_js_collLyα, :143-173); p_d (pd_Lyα, src/LymanAlphaDestruction.jl:43-97,
+ * at redshift z; 0 when destruction == 0); the continuum absorption α_c and flux opacity α_c,F of
+ * dust (densities below 1.1 ρd_floor ignored) and gas (Raman, OH, CH; Thomson in α_c,F); then the
+ * refresh of m1lya_prepare_subcycling (ΔνD, Hvoigt, ϕ_ν, α_Lyα = n_HI σ0(T), α_F = α_Lyα Hvoigt + α_c,F).
+ * chem [8][N_r, N_Z] = T, n_HI, n_H2, n_HII, n_HeI, n_e, n_OHx, n_CHx; rho_d [3, N_r, N_Z] (may be
+ * NULL); kappa = {κd_abs_Lyα, κd_F_Lyα} of silicate, graphite, PAH.  Needs m1lya_set_voigt_lut. */
+int m1lya_prepare_subcycling_chem(m1lya_ctx *ctx, const double *chem, const double *rho_d, const double *kappa,
+                                  double z, int ots, int destruction, int phi_all_bins);
 
 /* ---- reverse-mode gradient of the fused step ------------------------------
  * The reference has no adjoint; this entry point is what a ChainRules `rrule`
@@ -315,8 +381,20 @@ int  m1lya_batch_prepare_subcycling(m1lya_batch *b, int cloud, const double *T, 
 int  m1lya_batch_set_step(m1lya_batch *b, int cloud, double c_tilde, double dt, double z_center,
                           int bc_r_right, int bc_z_bot, int bc_z_top);
 /* n_steps steps of every cloud.  history (may be NULL) receives
- * [n_steps][n_clouds][8] step scalars laid out as m1lya_step's out. */
+ * [n_steps][n_clouds][8] step scalars laid out as m1lya_step's out, with ΔJ/J (as
+ * m1lya_run_until_steady defines it) as the last entry.  Returns
+ * M1LYA_ERR_NONFINITE, naming the first cloud and step, if a step wrote a
+ * non-finite J, H_r or H_Z. */
 int  m1lya_batch_run_steps(m1lya_batch *b, int n_steps, double *history);
+/* Steps every cloud until each one's ΔJ/J (as m1lya_run_until_steady defines it, in the
+ * cloud's FP32 state) has fallen below tol, or max_steps.  history (may be NULL) receives
+ * [*steps_done][n_clouds][8] step scalars (the last entry of each row is ΔJ/J); steady
+ * (may be NULL) receives [n_clouds] the step count after which each cloud was first below
+ * tol (0: never).  Clouds that are already steady keep stepping until the last one is.
+ * The FP32 state's rounding sets a floor of about 1e-7 under ΔJ/J, so tol should be
+ * above it. */
+int  m1lya_batch_run_until_steady(m1lya_batch *b, int max_steps, double tol, double *history,
+                                  int *steady, int *steps_done);
 /* [n_clouds][8] scalars of the last step. */
 int  m1lya_batch_scalars(m1lya_batch *b, double *out);
 /* [n_clouds][2] active-bin ranges the next step uses (0-based, inclusive). */

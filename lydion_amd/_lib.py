@@ -57,23 +57,32 @@ class LineConstants(C.Structure):
                 ("a_prefact", C.c_double), ("sigma0_prefact", C.c_double)]
 
 
+class RslInputs(C.Structure):
+    """m1lya_rsl_inputs"""
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("u_r", "u_z", "gamma_ad", "P", "rho", "rho_d", "xHII", "xH2",
+                                                     "T")] + [("kappa_abs", C.c_double * 3), ("lc", LineConstants)]
+
+
 EXPORTED = [
     "m1lya_abi_version", "m1lya_create", "m1lya_destroy", "m1lya_last_error", "m1lya_set_geometry",
     "m1lya_set_freq_grid", "m1lya_upload", "m1lya_download", "m1lya_fill", "m1lya_synchronize",
     "m1lya_find_active_bins", "m1lya_advect_lya", "m1lya_emission_absorption_update",
     "m1lya_update_freq_factors", "m1lya_frequency_update_all_cells", "m1lya_update_f_mf_l",
     "m1lya_update_f_mf_l_gravity", "m1lya_compute_f_esc", "m1lya_reset_drhou", "m1lya_update_e_lya",
-    "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_adjoint_run",
+    "m1lya_edge_count", "m1lya_edge_spectra", "m1lya_step", "m1lya_run_steps", "m1lya_run_until_steady",
+    "m1lya_adjoint_run",
     "m1lya_set_voigt_lut", "m1lya_prepare_subcycling",
     "m1lya_set_profiling", "m1lya_set_adjoint_mode",
     "m1lya_kernel_stats_count", "m1lya_kernel_stats", "m1lya_reset_kernel_stats", "m1lya_selftest_math",
     "m1lya_batch_create", "m1lya_batch_destroy", "m1lya_batch_last_error", "m1lya_batch_count",
     "m1lya_batch_set_geometry", "m1lya_batch_set_freq_grid", "m1lya_batch_set_scale", "m1lya_batch_get_scale",
     "m1lya_batch_upload", "m1lya_batch_download", "m1lya_batch_set_step", "m1lya_batch_run_steps",
+    "m1lya_batch_run_until_steady",
     "m1lya_batch_scalars", "m1lya_batch_active_bins", "m1lya_batch_set_tiles", "m1lya_batch_get_tiles",
     "m1lya_batch_autotune", "m1lya_batch_set_graph", "m1lya_batch_synchronize", "m1lya_batch_set_profiling",
     "m1lya_batch_kernel_stats_count", "m1lya_batch_kernel_stats", "m1lya_batch_reset_kernel_stats",
     "m1lya_batch_set_voigt_lut", "m1lya_batch_prepare_subcycling",
+    "m1lya_cfl_time_step", "m1lya_find_rsl", "m1lya_absorption_substep", "m1lya_prepare_subcycling_chem",
 ]
 
 _lib = None
@@ -117,6 +126,11 @@ def load() -> C.CDLL:
         "m1lya_edge_spectra": (i, [vp, _dp, _dp, sz]),
         "m1lya_step": (i, [vp, d, d, d, i, i, i, _dp]),
         "m1lya_run_steps": (i, [vp, i, d, d, d, i, i, i, _dp]),
+        "m1lya_run_until_steady": (i, [vp, i, d, d, d, d, i, i, i, _dp, C.POINTER(C.c_int)]),
+        "m1lya_cfl_time_step": (i, [vp, d, _dp, _dp, d, _dp]),
+        "m1lya_find_rsl": (i, [vp, d, d, C.POINTER(RslInputs), _dp]),
+        "m1lya_absorption_substep": (i, [vp, _dp, d, d, _dp, _dp, C.POINTER(C.c_int), _dp]),
+        "m1lya_prepare_subcycling_chem": (i, [vp, _dp, _dp, _dp, d, i, i, i]),
         "m1lya_adjoint_run": (i, [vp, i, d, d, d, i, i, i, _dp, _dp, i, _dp, _dp, _dp, _dp, _dp]),
         "m1lya_set_voigt_lut": (i, [vp, _dp, i, d, C.POINTER(LineConstants)]),
         "m1lya_prepare_subcycling": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp, i]),
@@ -140,6 +154,7 @@ def load() -> C.CDLL:
         "m1lya_batch_download": (i, [vp, i, i, _dp, sz]),
         "m1lya_batch_set_step": (i, [vp, i, d, d, d, i, i, i]),
         "m1lya_batch_run_steps": (i, [vp, i, _dp]),
+        "m1lya_batch_run_until_steady": (i, [vp, i, d, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "m1lya_batch_scalars": (i, [vp, _dp]),
         "m1lya_batch_active_bins": (i, [vp, ip]),
         "m1lya_batch_set_tiles": (i, [vp, i, i]),

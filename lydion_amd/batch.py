@@ -151,6 +151,16 @@ class M1LyaBatch:
         self._check(self.lib.m1lya_batch_run_steps(self.handle, int(n_steps), ptr(hist) if history else None))
         return hist
 
+    def run_until_steady(self, max_steps: int, tol: float):
+        """Steps until every cloud's ΔJ/J < tol (or max_steps).  Returns (history of the steps run,
+        (steps, n_clouds, 8) with ΔJ/J in column 7; per-cloud step count at which each became steady)."""
+        hist = np.empty((max(int(max_steps), 1), len(self), 8))
+        steady = (C.c_int * len(self))()
+        done = C.c_int(0)
+        self._check(self.lib.m1lya_batch_run_until_steady(self.handle, int(max_steps), float(tol), ptr(hist),
+                                                          steady, C.byref(done)))
+        return hist[:done.value].copy(), np.array(steady[:], dtype=int)
+
     def step(self) -> list[dict]:
         h = self.run_steps(1)
         return [dict(zip(STEP_FIELDS, row[:7])) for row in h[0]]

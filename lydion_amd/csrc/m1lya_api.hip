@@ -57,7 +57,12 @@ struct m1lya_ctx {
     // geometry
     double *A_r = nullptr, *A_z = nullptr, *dV = nullptr, *inv_dV = nullptr, *inv_r_cell = nullptr;
     double *rcom = nullptr, *zcom = nullptr, *ivr = nullptr, *ivz = nullptr;
-    bool geom_set = false;
+    double *r_if = nullptr, *z_if = nullptr;   // cell interfaces (RHD time-step control)
+    bool geom_set = false, interfaces_set = false;
+
+    // RHD sub-step and time-step control: staging of the host inputs (allocated on first use)
+    double* rhd = nullptr;                 // [16][ncell]
+    int* rhd_redo = nullptr;               // [ncell]
 
     // device-side coefficient refresh: Voigt tables, line constants, scratch inputs
     double *lut = nullptr, *dnu_rel = nullptr, *dnuC_rel = nullptr, *nHI = nullptr, *acF = nullptr;
@@ -75,6 +80,7 @@ struct m1lya_ctx {
     int col_blocks = 0, act_blocks = 0, pstride = 0, prows = 0;
     double *part = nullptr, *red = nullptr, *scalars = nullptr;
     int *krange = nullptr, *err_flag = nullptr;
+    StepCtl* ctl = nullptr;                // halt flag and executed-step count (m1lya_run_until_steady)
     double *edge_buf = nullptr;
     double *fg_r = nullptr, *fg_z = nullptr;
     double *history = nullptr;
@@ -276,7 +282,8 @@ void compute_krange(m1lya_ctx* c)
     c->krange_valid = true;
 }
 
-AdvectArgs advect_args(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top)
+AdvectArgs advect_args(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top,
+                       const int* halt = nullptr)
 {
     AdvectArgs a;
     a.nn = c->nn;
@@ -288,13 +295,14 @@ AdvectArgs advect_args(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, i
     a.bc_r_right = bc_r_right; a.bc_z_bot = bc_z_bot; a.bc_z_top = bc_z_top;
     a.hll = c->flux;
     a.krange = c->krange;
+    a.halt = halt;
     return a;
 }
 
 // Advection into the second buffer set, then swap roles.
-void run_advect(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top)
+void run_advect(m1lya_ctx* c, double cdt, int bc_r_right, int bc_z_bot, int bc_z_top, const int* halt = nullptr)
 {
-    AdvectArgs a = advect_args(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    AdvectArgs a = advect_args(c, cdt, bc_r_right, bc_z_bot, bc_z_top, halt);
     Prof p(c, "advect");
     launch_advect(c->stream, a);
     p.done();
@@ -321,9 +329,15 @@ ColumnArgs column_args(m1lya_ctx* c, int flags)
     a.alpha_floor = c->alpha_floor; a.recoil = c->recoil;
     a.c_tilde = 1.0; a.dt = 1.0; a.dw = 0.0;
     a.part = c->part; a.pstride = c->pstride; a.err_flag = c->err_flag;
+    a.halt = nullptr;
     return a;
 }
 
+// Reports (and clears) what the device error flag has collected since the last check:
+// a singular tridiagonal system (gtsv INFO > 0) or a non-finite J, H_r or H_Z written by a
+// fused step.  Steps queued without a host synchronisation report here, at the next call
+// that synchronises the context (m1lya_step with out, m1lya_run_steps, m1lya_synchronize,
+// m1lya_download).
 void check_err_flag(m1lya_ctx* c, const char* what)
 {
     int h = 0;
@@ -331,34 +345,49 @@ void check_err_flag(m1lya_ctx* c, const char* what)
     hipchk(c, hipStreamSynchronize(c->stream), "sync");
     if (h) {
         hipchk(c, hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream), "flag reset");
-        fail(c, M1LYA_ERR_NONFINITE, std::string(what) + ": singular tridiagonal system (gtsv INFO > 0)");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        std::string msg = std::string(what) + ":";
+        if (h & ERR_NONFINITE) msg += " non-finite J, H_r or H_Z in the state after a fused step;";
+        if (h & ERR_SINGULAR) msg += " singular tridiagonal system (gtsv INFO > 0);";
+        msg.pop_back();
+        fail(c, M1LYA_ERR_NONFINITE, msg);
     }
 }
 
+// Options of a fused step run by m1lya_run_until_steady: measure ΔJ/J, halt below tol.
+struct StepRun {
+    bool conv = false;
+    double tol = 0.0;
+    StepCtl* ctl = nullptr;
+};
+
 // tape_X1 (may be null): receives the advection output (J, H_r, H_Z), as replay_step records it
 void fused_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc_r_right, int bc_z_bot,
-                int bc_z_top, double* history_row, double* tape_X1 = nullptr)
+                int bc_z_top, double* history_row, double* tape_X1 = nullptr, const StepRun& run = StepRun{})
 {
     if (!c->krange_valid) compute_krange(c);
     const double cdt = c_tilde * dt;
-    run_advect(c, cdt, bc_r_right, bc_z_bot, bc_z_top);
+    const int* halt = run.ctl ? &run.ctl->halt : nullptr;
+    run_advect(c, cdt, bc_r_right, bc_z_bot, bc_z_top, halt);
     if (tape_X1) {
         const size_t b = c->n3 * sizeof(double);
         hipchk(c, hipMemcpyAsync(tape_X1, c->J, b, hipMemcpyDeviceToDevice, c->stream), "tape");
         hipchk(c, hipMemcpyAsync(tape_X1 + c->n3, c->Hr, b, hipMemcpyDeviceToDevice, c->stream), "tape");
         hipchk(c, hipMemcpyAsync(tape_X1 + 2 * c->n3, c->Hz, b, hipMemcpyDeviceToDevice, c->stream), "tape");
     }
-    ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED);
+    // in a fused step the *_old arrays are not read as such: J_old is the step's input state
+    ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED | (run.conv ? COL_CONV : 0));
     a.c_tilde = c_tilde; a.dt = dt; a.dw = 1.0 / cdt; a.z_center = z_center; a.bc_z_bot = bc_z_bot;
+    a.halt = halt;
     Prof p(c, "column");
     launch_column(c->stream, a, c->col_blocks);
     p.done();
     Prof p2(c, "reduce_cols");
-    launch_reduce_cols(c->stream, c->part, c->col_blocks, c->pstride, 2 * c->nn + 5, c->red);
+    launch_reduce_cols(c->stream, c->part, c->col_blocks, c->pstride, 2 * c->nn + kStepSums, c->red, halt);
     p2.done();
     Prof p3(c, "step_final");
     launch_step_final(c->stream, c->red, c->nn, c->kcore_min, c->kcore_max, bc_z_bot, c->krange, c->scalars,
-                      history_row);
+                      history_row, c->err_flag, run.ctl, run.tol, run.conv ? 1 : 0);
     p3.done();
     c->old_stale = true;
     c->fij_stale = true;
@@ -492,6 +521,7 @@ int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
         c->scalars = dalloc(c, 8);
         c->krange = ialloc(c, 2);
         c->err_flag = ialloc(c, 1);
+        c->ctl = (StepCtl*)ialloc(c, sizeof(StepCtl) / sizeof(int));
         int ne = (n_z - 2) + 2 * (n_r - 3);
         c->edge_buf = dalloc(c, (size_t)std::max(ne, 1) * n_nu * 2);
         hipchk(c, hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -545,6 +575,15 @@ int m1lya_set_geometry(m1lya_ctx* c, const m1lya_geometry* g)
         up(c->inv_r_cell, g->inv_r_cell, nr);
         up(c->rcom, g->rCoM, nr);
         up(c->zcom, g->ZCoM, nz);
+        if (g->r && g->Z) {
+            if (!c->r_if) {
+                c->r_if = dalloc(c, (size_t)nr + 1);
+                c->z_if = dalloc(c, (size_t)nz + 1);
+            }
+            up(c->r_if, g->r, (size_t)nr + 1);
+            up(c->z_if, g->Z, (size_t)nz + 1);
+            c->interfaces_set = true;
+        }
         // One interval set per interface: face 0 = inner boundary, face N = outer
         // boundary, face f in 1..N-1 uses the interior arrays at f-1
         // (src/AdvectJandH.jl:632-639, 803-812, 1092-1098, 1105-1114).
@@ -628,6 +667,7 @@ int m1lya_download(m1lya_ctx* c, int field, double* host, size_t count)
         hipchk(c, hipMemcpyAsync(host, fi.ptr, count * sizeof(double), hipMemcpyDeviceToHost, c->stream),
                "download");
         hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        check_err_flag(c, "step");
     });
 }
 
@@ -920,6 +960,191 @@ int m1lya_run_steps(m1lya_ctx* c, int n_steps, double c_tilde, double dt, double
     });
 }
 
+int m1lya_run_until_steady(m1lya_ctx* c, int max_steps, double tol, double c_tilde, double dt, double z_center,
+                           int bc_r_right, int bc_z_bot, int bc_z_top, double* history, int* steps_done)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        check_bc(c, bc_r_right, "BC_r_right");
+        check_bc(c, bc_z_bot, "BC_Z_bot");
+        check_bc(c, bc_z_top, "BC_Z_top");
+        if (max_steps < 0) fail(c, M1LYA_ERR_ARG, "max_steps < 0");
+        if (!(tol > 0.0)) fail(c, M1LYA_ERR_ARG, "tol must be > 0");
+        if (!(c_tilde > 0.0) || !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde and dt must be > 0");
+        // Steps are queued in chunks; after the step whose ΔJ/J falls below tol, k_step_final
+        // sets the halt flag and the rest of the chunk's launches return at once.
+        const int chunk = 256;
+        if ((size_t)chunk * 8 > c->history_cap) {
+            if (c->history) hipchk(c, hipFree(c->history), "hipFree");
+            c->history = nullptr;
+            hipchk(c, hipMalloc(&c->history, (size_t)chunk * 8 * sizeof(double)), "hipMalloc");
+            c->history_cap = (size_t)chunk * 8;
+        }
+        StepRun run;
+        run.conv = true;
+        run.tol = tol;
+        run.ctl = c->ctl;
+        int done = 0;
+        bool halted = false;
+        while (done < max_steps && !halted) {
+            const int n = std::min(chunk, max_steps - done);
+            const StepCtl zero{0, 0};
+            hipchk(c, hipMemcpyAsync(c->ctl, &zero, sizeof(StepCtl), hipMemcpyHostToDevice, c->stream), "ctl");
+            for (int s = 0; s < n; ++s)
+                fused_step(c, c_tilde, dt, z_center, bc_r_right, bc_z_bot, bc_z_top, c->history + (size_t)s * 8,
+                           nullptr, run);
+            StepCtl h{0, 0};
+            hipchk(c, hipMemcpyAsync(&h, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream), "ctl");
+            hipchk(c, hipStreamSynchronize(c->stream), "sync");
+            // every launched step swapped the buffer roles on the host; the halted ones did not run
+            if ((n - h.steps) & 1) {
+                std::swap(c->J, c->Jo);
+                std::swap(c->Hr, c->Hro);
+                std::swap(c->Hz, c->Hzo);
+            }
+            if (history && h.steps > 0)
+                hipchk(c, hipMemcpyAsync(history + (size_t)done * 8, c->history, (size_t)h.steps * 8 * sizeof(double),
+                                         hipMemcpyDeviceToHost, c->stream),
+                       "history");
+            done += h.steps;
+            halted = h.halt != 0;
+            check_err_flag(c, "run_until_steady");
+        }
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        if (steps_done) *steps_done = done;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+double* rhd_stage(m1lya_ctx* c, int slot, const double* host, size_t n)
+{
+    if (!c->rhd) {
+        c->rhd = dalloc(c, (size_t)16 * c->ncell);
+        c->rhd_redo = ialloc(c, c->ncell);
+    }
+    double* d = c->rhd + (size_t)slot * c->ncell;
+    if (host) hipchk(c, hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
+    return d;
+}
+
+double rhd_scan(m1lya_ctx* c, RhdScanArgs& a)
+{
+    if (!c->interfaces_set) fail(c, M1LYA_ERR_STATE, "cell interfaces r, Z not set (m1lya_set_geometry)");
+    a.nr = c->nr;
+    a.nz = c->nz;
+    a.r = c->r_if;
+    a.Z = c->z_if;
+    const int nb = (int)std::min<size_t>(std::max<size_t>((c->ncell + 255) / 256, 1), 1024);
+    Prof p(c, a.mode ? "rsl_scan" : "cfl_scan");
+    launch_rhd_scan(c->stream, a, nb, c->part, c->red);
+    p.done();
+    double v = 0.0;
+    hipchk(c, hipMemcpyAsync(&v, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream), "scan");
+    hipchk(c, hipStreamSynchronize(c->stream), "sync");
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int m1lya_cfl_time_step(m1lya_ctx* c, double c_tilde, const double* u_r, const double* u_z, double cfl, double* dt)
+{
+    return guarded(c, [&] {
+        if (!u_r || !u_z || !dt) fail(c, M1LYA_ERR_ARG, "NULL argument");
+        if (!(c_tilde > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde must be > 0");
+        RhdScanArgs a{};
+        a.mode = 0;
+        a.c_tilde = c_tilde;
+        a.u_r = rhd_stage(c, 0, u_r, c->ncell);
+        a.u_z = rhd_stage(c, 1, u_z, c->ncell);
+        *dt = cfl * rhd_scan(c, a);
+    });
+}
+
+int m1lya_find_rsl(m1lya_ctx* c, double c_old, double c_min, const m1lya_rsl_inputs* in, double* c_new)
+{
+    return guarded(c, [&] {
+        if (!in || !c_new || !in->u_r || !in->u_z || !in->gamma_ad || !in->P || !in->rho || !in->rho_d ||
+            !in->xHII || !in->xH2 || !in->T)
+            fail(c, M1LYA_ERR_ARG, "NULL argument");
+        const size_t n = c->ncell;
+        RhdScanArgs a{};
+        a.mode = 1;
+        a.u_r = rhd_stage(c, 0, in->u_r, n);
+        a.u_z = rhd_stage(c, 1, in->u_z, n);
+        a.gamma_ad = rhd_stage(c, 2, in->gamma_ad, n);
+        a.P = rhd_stage(c, 3, in->P, n);
+        a.rho = rhd_stage(c, 4, in->rho, n);
+        a.xHII = rhd_stage(c, 5, in->xHII, n);
+        a.xH2 = rhd_stage(c, 6, in->xH2, n);
+        a.T = rhd_stage(c, 7, in->T, n);
+        a.rho_d = rhd_stage(c, 8, in->rho_d, 3 * n);
+        a.pd = c->pd;
+        a.kS = in->kappa_abs[0];
+        a.kG = in->kappa_abs[1];
+        a.kP = in->kappa_abs[2];
+        a.lc = LineConst{in->lc.b_prefact, in->lc.nu_over_c, in->lc.a_prefact, in->lc.sigma0_prefact};
+        // X / m_H of src/ConstantsAndFunctions.jl:124-128,167
+        const double Y = 0.247, D_H = 2.53e-5, amu = 1.660539069e-24;
+        const double mH = 1.0080 * amu, mD = 2.014 * amu;
+        const double X_D = D_H * (1 - Y) * mD / mH;
+        const double X = 1 - Y - X_D;
+        a.X_over_mH = X / mH;
+        double v = rhd_scan(c, a);
+        // clamp and rate limits of src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:706-721
+        v = std::min(std::max(v, c_min), 2.99792458e10);
+        if (v < c_old) v = std::max(v, 0.75 * c_old);
+        if (v > c_old) v = std::min(v, 4.0 * c_old);
+        *c_new = v;
+    });
+}
+
+int m1lya_absorption_substep(m1lya_ctx* c, const double* dt_cell, double dt, double c_tilde, const double* rho_d,
+                             const double* kappa_F, int* redo, double* cell_out)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (!(c_tilde > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde must be > 0");
+        if (!dt_cell && !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "dt must be > 0 when dt_cell is NULL");
+        if (rho_d && !kappa_F) fail(c, M1LYA_ERR_ARG, "kappa_F is NULL");
+        materialize_old(c);
+        const size_t n = c->ncell;
+        double* d_dt = rhd_stage(c, 0, dt_cell, n);
+        if (!dt_cell) {
+            std::vector<double> h(n, dt);
+            hipchk(c, hipMemcpyAsync(d_dt, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream), "dt");
+            hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        }
+        AbsorbArgs a{};
+        a.nn = c->nn;
+        a.ncell = (int)n;
+        a.dt = d_dt;
+        a.c_tilde = c_tilde;
+        a.J_old = c->Jo; a.Hr_old = c->Hro; a.Hz_old = c->Hzo;
+        a.J = c->J; a.Hr = c->Hr; a.Hz = c->Hz;
+        a.phi = c->phi; a.Hv = c->Hv; a.aF = c->aF;
+        a.alpha = c->alpha; a.alpha_c = c->alpha_c; a.pd = c->pd; a.js = c->js; a.e_lya = c->e_lya;
+        a.dnu = c->dnu;
+        a.rho_d = rho_d ? rhd_stage(c, 1, rho_d, 3 * n) : nullptr;
+        if (kappa_F) { a.kS = kappa_F[0]; a.kG = kappa_F[1]; a.kP = kappa_F[2]; }
+        a.redo = c->rhd_redo;
+        a.out = c->rhd + 4 * n;
+        Prof p(c, "absorb");
+        launch_absorb(c->stream, a);
+        p.done();
+        if (redo) hipchk(c, hipMemcpyAsync(redo, c->rhd_redo, n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "redo");
+        if (cell_out)
+            hipchk(c, hipMemcpyAsync(cell_out, a.out, 12 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                   "cell_out");
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        c->krange_valid = false;
+    });
+}
+
 int m1lya_set_voigt_lut(m1lya_ctx* c, const double* luts, int nx, double xmax, const m1lya_line_constants* lc)
 {
     return guarded(c, [&] {
@@ -975,6 +1200,41 @@ int m1lya_prepare_subcycling(m1lya_ctx* c, const double* T, const double* n_HI, 
         p.done();
         c->fij_stale = true;
         hipchk(c, hipStreamSynchronize(st), "sync");
+    });
+}
+
+int m1lya_prepare_subcycling_chem(m1lya_ctx* c, const double* chem, const double* rho_d, const double* kappa,
+                                  double z, int ots, int destruction, int phi_all_bins)
+{
+    return guarded(c, [&] {
+        require_setup(c);
+        if (!c->lut) fail(c, M1LYA_ERR_STATE, "prepare_subcycling_chem: Voigt tables not set (m1lya_set_voigt_lut)");
+        if (!chem || !kappa) fail(c, M1LYA_ERR_ARG, "prepare_subcycling_chem: chem and kappa are required");
+        const size_t n = c->ncell;
+        SourceArgs a{};
+        a.ncell = (int)n;
+        a.ots = ots ? 1 : 0;
+        a.destruction = destruction ? 1 : 0;
+        a.z = z;
+        a.chem = rhd_stage(c, 0, chem, 8 * n);
+        a.rho_d = rho_d ? rhd_stage(c, 8, rho_d, 3 * n) : nullptr;
+        for (int m = 0; m < 6; ++m) a.kappa[m] = kappa[m];
+        a.js = c->js; a.alpha_c = c->alpha_c; a.acF = c->acF; a.pd = c->pd; a.T = c->T; a.nHI = c->nHI;
+        Prof p0(c, "sources");
+        launch_sources(c->stream, a);
+        p0.done();
+        RefreshArgs r;
+        r.nn = c->nn; r.ncell = (int)n; r.kmin = c->kcore_min; r.kmax = c->kcore_max;
+        r.phi_all_bins = phi_all_bins ? 1 : 0;
+        r.lut = c->lut; r.nx = c->lut_nx; r.xmax = c->lut_xmax; r.lc = c->lc;
+        r.dnu_rel = c->dnu_rel; r.dnuC_rel = c->dnuC_rel; r.dnu = c->dnu;
+        r.T = c->T; r.nHI = c->nHI; r.acF = c->acF;
+        r.Hv = c->Hv; r.phi = c->phi; r.aF = c->aF; r.alpha = c->alpha; r.dnuD = c->dnuD;
+        Prof p(c, "refresh");
+        launch_refresh(c->stream, r);
+        p.done();
+        c->fij_stale = true;
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
     });
 }
 

@@ -56,7 +56,7 @@ constexpr double kHeps2 = 1e-200;        // |H|^2 floor of the cell-centred clos
 constexpr double kEpsActive = 1e-10;     // FindActiveBins threshold
 constexpr double kPmin = 1e-3, kPmax = 40.0;
 constexpr int kMaxBins = 128;
-constexpr int kNScal = 5;                        // MF, L, net, out, esc
+constexpr int kNScal = 7;                        // MF, L, net, out, esc, ΔJ sum, non-finite count
 constexpr int kPartStride = 2 * kMaxBins + 8;    // e1[N_ν], e10[N_ν], scalars
 constexpr int kMaxWaves = 8;
 constexpr int kSlices = 16;                      // row slices of the first reduction stage
@@ -106,6 +106,13 @@ struct Seg {
 struct Ctrl {
     double* hist;                  // [cap][n_clouds][8]
     int cap, step, next;           // history cursor of the running step / of the next step
+    int halt;                      // set once every cloud is steady: later launches return at once
+    int live;                      // k_reduce's view of halt, read by k_final (halt may change while
+                                   // k_final runs, since one of its blocks sets it)
+    int nsteady;                   // clouds whose ΔJ/J has fallen below tol
+    int bad, bad_step;             // first cloud (+1) with a non-finite J, H_r or H_Z, and its step
+    double tol;                    // steady-state tolerance of ΔJ/J (0: no stop)
+    int* steady;                   // [n_clouds]: 1 + the step at which the cloud became steady, else 0
 };
 
 struct Arrays {
@@ -355,8 +362,9 @@ template <bool kShear>
 __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restrict__ clouds, Arrays A,
                                                             const Seg* __restrict__ segs, int nblk, int wpb,
                                                             int parity, const int* __restrict__ krange,
-                                                            double* __restrict__ part)
+                                                            double* __restrict__ part, const Ctrl* __restrict__ ctrl)
 {
+    if (ctrl->halt) return;
     extern __shared__ __align__(16) unsigned char smem[];
     BinRec* sbin = reinterpret_cast<BinRec*>(smem);
     double (*sh)[kPartStride] = reinterpret_cast<double (*)[kPartStride]>(sbin + kMaxBins);
@@ -373,7 +381,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
 
     // per-wavefront compensated sums of M_F, L, radial net/out and f_esc, kept by lane 0
     // in LDS (s, c pairs) rather than in registers of every lane
-    enum { S_MF = 0, S_L, S_NET, S_OUT, S_ESC };
+    enum { S_MF = 0, S_L, S_NET, S_OUT, S_ESC, S_CONV, S_BAD };
     if (lane < 2 * kNScal) sacc[wave][lane] = 0.0;
     auto kadd = [&](int m, double x) {
         if (lane == 0) {
@@ -456,6 +464,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         FL[q][0] = FL[q][1] = FL[q][2] = 0.f;
         kimR[q] = 0.f;
     }
+
+    // per lane: Σ Δν ΔV |J_new - J_start| (the cloud's ΔJ/J) and the non-finite outputs
+    double conv = 0.0;
+    int bad = 0;
 
     // All wavefronts of the block march over the same cells [i0, i1) in lockstep;
     // i = i0-1 only evaluates the left r interface of cell i0.
@@ -832,12 +844,15 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 h_z *= s;
             }
             const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
+            // the floor would hide a NaN in J (NaN > Jeps is false): test the solve's output
+            bad |= !(__builtin_isfinite((float)jn) && __builtin_isfinite(hrf) && __builtin_isfinite(hzf));
             const size_t idx = base + k;
             auto& cw = cloud_now();
             cw.sJ[parity ^ 1][idx] = Jf;
             cw.sHr[parity ^ 1][idx] = hrf;
             cw.sHz[parity ^ 1][idx] = hzf;
             const double Jd = Jf, dk = bins[k].dnu;
+            conv += dk * (double)fabsf(Jf - wJ[q][1]) * cr.dV;
             sh[wave][k] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
             sh[wave][nn + k] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
             // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
@@ -852,6 +867,12 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
     // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
     const Cloud& cb = clouds[cidx];
     const int nnb = cb.nn;
+    {
+        const double cs = wave_sum64(conv);
+        const double nb = (double)__popcll(__ballot(bad != 0));
+        kadd(S_CONV, cs * cb.J0);
+        kadd(S_BAD, nb);
+    }
     for (int t = lane; t < 2 * nnb; t += 64) sh[wave][t] *= cb.J0;
     if (lane == 0) {
         for (int m = 0; m < kNScal; ++m) sh[wave][2 * nnb + m] = sacc[wave][2 * m] + sacc[wave][2 * m + 1];
@@ -922,7 +943,12 @@ __global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds
                                                 double* __restrict__ red, Ctrl* __restrict__ ctrl)
 {
     __shared__ double ss[4][64], sc[4][64];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) ctrl->step = ctrl->next;
+    const bool halted = ctrl->halt != 0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+        ctrl->live = !halted;
+        if (!halted) ctrl->step = ctrl->next;
+    }
+    if (halted) return;
     const Cloud& cl = clouds[blockIdx.y];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + lane;
@@ -957,6 +983,7 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
     __shared__ double col[kPartStride];
     __shared__ double tree[256];
     __shared__ int kmn_s, kmx_s;
+    if (!ctrl->live) return;
     const int c = blockIdx.x;
     const Cloud& cl = clouds[c];
     const int nn = cl.nn, ncol = 2 * nn + kNScal, t = threadIdx.x;
@@ -997,11 +1024,18 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
             const double Lsim = (cl.bc_z_bot == M1LYA_REFLECT) ? 0.5 * Ltot : Ltot;
             fesc = (Lsim > 0.0) ? es / Lsim : 0.0;
         }
-        const double v[8] = {MF, nt, ow, Ltot, fesc, (double)used0, (double)used1, 0.0};
+        // ΔJ/J = Σ Δν ΔV |J_new - J_start| / E_tot (E_tot: the energy FindActiveBins measures)
+        const double rel = (Etot > 0.0) ? col[2 * nn + 5] / Etot : 0.0;
+        const double v[8] = {MF, nt, ow, Ltot, fesc, (double)used0, (double)used1, rel};
         for (int m = 0; m < 8; ++m) scalars[(size_t)c * 8 + m] = v[m];
         const int step = ctrl->step;
         if (ctrl->hist && step < ctrl->cap)
             for (int m = 0; m < 8; ++m) ctrl->hist[((size_t)step * n + c) * 8 + m] = v[m];
+        if (col[2 * nn + 6] != 0.0 && atomicCAS(&ctrl->bad, 0, c + 1) == 0) ctrl->bad_step = step;
+        if (ctrl->tol > 0.0 && rel < ctrl->tol && ctrl->steady[c] == 0) {
+            ctrl->steady[c] = step + 1;
+            if (atomicAdd(&ctrl->nsteady, 1) == n - 1) ctrl->halt = 1;
+        }
         if (c == 0) ctrl->next = step + 1;
     }
     krange[2 * c] = min(kmn_s, cl.kcore_min);
@@ -1082,7 +1116,8 @@ struct m1lya_batch {
     int* krange = nullptr;
     double* scalars = nullptr;
     Ctrl* ctrl = nullptr;
-    Ctrl h_ctrl{nullptr, 0, 0, 0};
+    Ctrl h_ctrl{nullptr, 0, 0, 0, 0, 1, 0, 0, 0, 0.0, nullptr};
+    int* steady = nullptr;                 // [n]: Ctrl::steady
     double* hist = nullptr;
     int hist_cap = 0;
 
@@ -1236,10 +1271,11 @@ void push_records(m1lya_batch* b, int cloud)
          "cell records");
 }
 
-void set_ctrl(m1lya_batch* b, double* hist, int cap)
+void set_ctrl(m1lya_batch* b, double* hist, int cap, double tol = 0.0)
 {
-    b->h_ctrl = Ctrl{hist, cap, 0, 0};
+    b->h_ctrl = Ctrl{hist, cap, 0, 0, 0, 1, 0, 0, 0, tol, b->steady};
     bchk(b, hipMemcpy(b->ctrl, &b->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice), "ctrl");
+    bchk(b, hipMemset(b->steady, 0, sizeof(int) * b->n), "steady");
 }
 
 hipEvent_t pool_event(m1lya_batch* b)
@@ -1306,10 +1342,10 @@ void launch_step_kernels(m1lya_batch* b, int parity)
         const size_t lds = step_lds_bytes(b->wpb);
         if (b->shear)
             hipLaunchKernelGGL(k_step<true>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
-                               b->nblk, b->wpb, parity, b->krange, b->part);
+                               b->nblk, b->wpb, parity, b->krange, b->part, b->ctrl);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
-                               b->nblk, b->wpb, parity, b->krange, b->part);
+                               b->nblk, b->wpb, parity, b->krange, b->part, b->ctrl);
         p.done();
     }
     {
@@ -1367,18 +1403,37 @@ void capture_graphs(m1lya_batch* b)
     b->profiling = prof;
 }
 
-void run_steps(m1lya_batch* b, int n_steps, double* history)
+// Reports a non-finite J, H_r or H_Z that a step of the last run wrote (k_final records the
+// first cloud and step), then clears the device controls.
+void check_nonfinite(m1lya_batch* b, const char* what)
 {
-    require_ready(b);
-    sync_clouds(b);
-    if (!b->krange_valid) compute_krange(b);
-    if (history && n_steps > b->hist_cap) {
+    Ctrl h{};
+    bchk(b, hipMemcpy(&h, b->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost), "ctrl");
+    if (h.bad) {
+        set_ctrl(b, nullptr, 0);
+        bfail(b, M1LYA_ERR_NONFINITE, std::string(what) + ": cloud " + std::to_string(h.bad - 1) +
+                                          ": non-finite J, H_r or H_Z after step " + std::to_string(h.bad_step) +
+                                          " of the call");
+    }
+}
+
+void ensure_hist(m1lya_batch* b, int n_steps)
+{
+    if (n_steps > b->hist_cap) {
         bfree(b, b->hist);
         b->hist_cap = std::max(n_steps, 64);
         b->hist = balloc<double>(b, (size_t)b->hist_cap * b->n * 8);
     }
+}
+
+void run_steps(m1lya_batch* b, int n_steps, double* history)
+{
+    require_ready(b);
+    sync_clouds(b);
+    if (history) ensure_hist(b, n_steps);
     bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     set_ctrl(b, history ? b->hist : nullptr, history ? b->hist_cap : 0);
+    if (!b->krange_valid) compute_krange(b);
     const bool graph = b->use_graph && !b->profiling;
     if (graph) capture_graphs(b);
     for (int s = 0; s < n_steps; ++s) {
@@ -1394,6 +1449,49 @@ void run_steps(m1lya_batch* b, int n_steps, double* history)
              "history download");
     }
     bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    check_nonfinite(b, "run_steps");
+}
+
+// Steps until every cloud's ΔJ/J has fallen below tol (k_final then sets the halt flag and
+// the rest of the queued launches return at once), or max_steps.  steady[c] receives the
+// number of steps after which cloud c first was below tol (0: never).
+int run_until_steady(m1lya_batch* b, int max_steps, double tol, double* history, int* steady)
+{
+    require_ready(b);
+    sync_clouds(b);
+    if (history) ensure_hist(b, max_steps);
+    bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    set_ctrl(b, history ? b->hist : nullptr, history ? b->hist_cap : 0, tol);
+    if (!b->krange_valid) compute_krange(b);
+    const bool graph = b->use_graph && !b->profiling;
+    if (graph) capture_graphs(b);
+    const int chunk = 64;
+    int launched = 0;
+    Ctrl h{};
+    while (launched < max_steps) {
+        const int n = std::min(chunk, max_steps - launched);
+        for (int s = 0; s < n; ++s) {
+            if (graph)
+                bchk(b, hipGraphLaunch(b->gexec[b->parity], b->stream), "hipGraphLaunch");
+            else
+                launch_step_kernels(b, b->parity);
+            b->parity ^= 1;
+        }
+        launched += n;
+        bchk(b, hipMemcpyAsync(&h, b->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, b->stream), "ctrl");
+        bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        if (h.halt || h.bad) break;
+    }
+    // every launched step flipped the buffer parity on the host; the halted ones did not run
+    const int done = h.next;
+    if ((launched - done) & 1) b->parity ^= 1;
+    if (history && done > 0)
+        bchk(b, hipMemcpy(history, b->hist, (size_t)done * b->n * 8 * sizeof(double), hipMemcpyDeviceToHost),
+             "history download");
+    if (steady) bchk(b, hipMemcpy(steady, b->steady, sizeof(int) * b->n, hipMemcpyDeviceToHost), "steady");
+    check_nonfinite(b, "run_until_steady");
+    set_ctrl(b, nullptr, 0);
+    return done;
 }
 
 void up_f32(m1lya_batch* b, float* dst, const double* src, size_t n, double scale)
@@ -1470,6 +1568,7 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
         b->krange = balloc<int>(b, (size_t)n_clouds * 2);
         b->scalars = balloc<double>(b, (size_t)n_clouds * 8);
         b->ctrl = balloc<Ctrl>(b, 1);
+        b->steady = balloc<int>(b, (size_t)n_clouds);
         build_tiles(b);
     });
     if (rc != M1LYA_OK) {
@@ -1846,6 +1945,18 @@ int m1lya_batch_run_steps(m1lya_batch* b, int n_steps, double* history)
         if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
         if (n_steps < 0) bfail(b, M1LYA_ERR_ARG, "n_steps < 0");
         run_steps(b, n_steps, history);
+    });
+}
+
+int m1lya_batch_run_until_steady(m1lya_batch* b, int max_steps, double tol, double* history, int* steady,
+                                 int* steps_done)
+{
+    return bguard(b, [&] {
+        if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
+        if (max_steps < 0) bfail(b, M1LYA_ERR_ARG, "max_steps < 0");
+        if (!(tol > 0.0)) bfail(b, M1LYA_ERR_ARG, "tol must be > 0");
+        const int done = run_until_steady(b, max_steps, tol, history, steady);
+        if (steps_done) *steps_done = done;
     });
 }
 

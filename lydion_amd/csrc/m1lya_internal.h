@@ -51,6 +51,7 @@ struct AdvectArgs {
     double cdt;
     int bc_r_right, bc_z_bot, bc_z_top, hll;
     const int* krange;
+    const int* halt;      // may be null; when *halt != 0 the launch does nothing
 };
 
 enum ColumnFlags {
@@ -59,8 +60,15 @@ enum ColumnFlags {
     COL_STORE_F = 4,  // ... and stored to Fij_kp/k/km
     COL_SOLVE = 8,    // tridiagonal frequency solve
     COL_HCORR = 16,   // J floor + H rescale + causality
-    COL_FUSED = 32    // fused step: no *_old arrays, per-block reductions
+    COL_FUSED = 32,   // fused step: no *_old arrays, per-block reductions
+    COL_CONV = 64     // fused step: also Σ Δν ΔV |J_new - J_start| (J_start read from J_old)
 };
+
+// Columns of a fused-step partial row after the 2 N_ν active-bin energies.
+enum StepSums { SUM_MF = 0, SUM_L, SUM_NET, SUM_OUT, SUM_ESC, SUM_CONV, SUM_BAD, kStepSums };
+
+// Bits of the context's device error flag.
+enum ErrBits { ERR_SINGULAR = 1, ERR_NONFINITE = 2 };
 
 struct ColumnArgs {
     int nn, nr, nz, flags;
@@ -106,6 +114,13 @@ struct ColumnArgs {
     double* part;
     int pstride;
     int* err_flag;
+    const int* halt;      // may be null; when *halt != 0 the launch does nothing
+};
+
+// Device-side control of a fused step: halt flag and count of executed steps.
+struct StepCtl {
+    int halt;             // set by k_step_final once ΔJ/J < tol; later launches return at once
+    int steps;            // fused steps executed since the host last cleared it
 };
 
 struct MfArgs {
@@ -215,16 +230,23 @@ void launch_adj_axpy_cell(hipStream_t s, double* y, const double* dV, double a, 
 
 void launch_active_partial(hipStream_t s, const double* J, const double* dnu, const double* dV, int nn,
                            int ncells, int nblocks, double* part, int pstride);
-void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out);
+void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out,
+                        const int* halt = nullptr);
 void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange);
 void launch_advect(hipStream_t s, const AdvectArgs& a);
 int column_width();
-// Blocks of the column kernel for ngroups column groups of an N_ν-bin grid: as many as the
-// GPU holds at once (one launch wave), at most ngroups.
+// Blocks of the column kernel for ngroups column groups: a fixed count (at most ngroups),
+// so that the partition of the fused reductions, and with it their bits, does not depend
+// on the device.  2048 is one launch wave on MI355X (8 blocks per CU, 256 CUs).
+constexpr int kColumnBlocks = 2048;
 int column_blocks(int nn, int ngroups);
 void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks);
+// Step scalars {M_F, net, out, L, f_esc, kmin, kmax, ΔJ/J}; ΔJ/J is 0 unless the column
+// kernel ran with COL_CONV.  ctl may be null (no halt, no count); tol > 0 sets ctl->halt
+// once ΔJ/J < tol.  A non-finite count sets ERR_NONFINITE in err_flag.
 void launch_step_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int bc_z_bot,
-                       int* krange, double* scalars, double* history);
+                       int* krange, double* scalars, double* history, int* err_flag, StepCtl* ctl,
+                       double tol, int conv);
 void launch_mf_partial(hipStream_t s, const MfArgs& a, int nblocks);
 void launch_fesc_partial(hipStream_t s, const double* Hr, const double* Hz, const double* A_r,
                          const double* A_z, const double* dnu, int nn, int nr, int nz, int bc_z_bot,
@@ -267,5 +289,71 @@ struct RefreshArgs {
 };
 
 void launch_refresh(hipStream_t s, const RefreshArgs& a);
+
+// ---- RHD time-step control: GetRHDCFLTimeStep / FindRSL_LyαRT per-cell terms ----
+struct RhdScanArgs {
+    int nr, nz, mode;       // mode 0: min over cells of 0.5 min(Δr, ΔZ) / (c̃ + |u|); 1: max of the RSL floor
+    const double* r;        // [nr+1] interfaces
+    const double* Z;        // [nz+1]
+    const double* u_r;      // [nz][nr] cell velocities
+    const double* u_z;
+    const double* gamma_ad; // mode 1 only
+    const double* P;
+    const double* rho;
+    const double* rho_d;    // [nz][nr][3]
+    const double* xHII;
+    const double* xH2;
+    const double* T;
+    const double* pd;
+    double kS, kG, kP;      // κd_abs_Lyα of silicate, graphite, PAH dust
+    double c_tilde;
+    LineConst lc;
+    double X_over_mH;
+};
+// Writes the reduced value to out[0] (deterministic: min / max are exact); part holds nblocks doubles.
+void launch_rhd_scan(hipStream_t s, const RhdScanArgs& a, int nblocks, double* part, double* out);
+
+// ---- the Lyα block of the RHD absorption sub-step (AbsorptionStep) ----
+struct AbsorbArgs {
+    int nn, ncell;
+    const double* dt;       // [ncell] sub-step of each cell
+    double c_tilde;
+    const double* J_old;
+    const double* Hr_old;
+    const double* Hz_old;
+    double* J;
+    double* Hr;
+    double* Hz;
+    const double* phi;
+    const double* Hv;
+    const double* aF;
+    const double* alpha;
+    const double* alpha_c;
+    const double* pd;
+    const double* js;
+    const double* e_lya;    // e_Lyα before the sub-step
+    const double* dnu;
+    const double* rho_d;    // [ncell][3] or null
+    double kS, kG, kP;      // κd_F_Lyα of silicate, graphite, PAH dust
+    int* redo;              // [ncell]
+    double* out;            // [12][ncell]
+};
+void launch_absorb(hipStream_t s, const AbsorbArgs& a);
+
+// ---- Lyα source terms of PrepareLyαSubCycling! from the chemical state ----
+struct SourceArgs {
+    int ncell, ots, destruction;
+    double z;
+    const double* chem;     // [8][ncell]: T, n_HI, n_H2, n_HII, n_HeI, n_e, n_OHx, n_CHx
+    const double* rho_d;    // [ncell][3] or null
+    double kappa[6];        // κd_abs_Lyα, κd_F_Lyα of silicate, graphite, PAH
+    double* js;             // outputs [ncell]
+    double* alpha_c;
+    double* acF;
+    double* pd;
+    double* T;
+    double* nHI;
+};
+void launch_sources(hipStream_t s, const SourceArgs& a);
 
 }  // namespace m1

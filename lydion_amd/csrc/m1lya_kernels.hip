@@ -3,6 +3,8 @@
 // Layout: spectral arrays are [j][i][k] (k = frequency bin fastest), exactly the
 // Julia column-major [k,i,j] of the reference, so a wavefront of 64 lanes on
 // consecutive k reads 512 contiguous bytes per FP64 field.
+#include <algorithm>
+
 #include "m1lya_internal.h"
 
 namespace m1 {
@@ -62,9 +64,11 @@ __global__ __launch_bounds__(256) void k_active_partial(const double* __restrict
 // Column sums of a [nrows][stride] partial array: out[col] = Σ_rows part[row][col].
 // One block per column, fixed-shape tree -> deterministic.
 __global__ __launch_bounds__(256) void k_reduce_cols(const double* __restrict__ part, int nrows,
-                                                      int stride, double* __restrict__ out)
+                                                      int stride, double* __restrict__ out,
+                                                      const int* __restrict__ halt)
 {
     __shared__ double s[256];
+    if (halt && *halt) return;
     const int col = blockIdx.x;
     double a = 0.0;
     for (int r = threadIdx.x; r < nrows; r += 256) a += part[(size_t)r * stride + col];
@@ -175,6 +179,7 @@ __global__ __launch_bounds__(64 * BI * BJ) void k_advect(AdvectArgs a)
     // value at f = 0); Z faces of its columns: F_J, F_Hr, F_HZ
     __shared__ double rF[BJ][BI + 1][5][64];
     __shared__ double zF[BJ + 1][BI][3][64];
+    if (a.halt && *a.halt) return;
     const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
     const int lane = threadIdx.x, il = threadIdx.y, jl = threadIdx.z;
     const int k = blockIdx.x * 64 + lane;
@@ -351,6 +356,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
     const int flags = a.flags;
     const bool EMIT = flags & COL_EMIT, FREQ = flags & COL_FREQ, STOREF = flags & COL_STORE_F;
     const bool SOLVE = flags & COL_SOLVE, HCORR = flags & COL_HCORR, FUSED = flags & COL_FUSED;
+    const bool CONV = flags & COL_CONV;
+    if (a.halt && *a.halt) return;
     // [column][row] with an odd row stride: conflict-free for both the lanes-over-k
     // writes (consecutive rows) and the lane-per-column sweep (stride S, S odd).
     const int S = col_stride(nn);
@@ -369,6 +376,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
 #pragma unroll
     for (int kk = 0; kk < KPL; ++kk) { e1[kk] = 0.0; e10[kk] = 0.0; }
     double acc_mf = 0.0, acc_L = 0.0, acc_net = 0.0, acc_out = 0.0, acc_esc = 0.0;
+    double conv = 0.0;   // per lane: Σ Δν ΔV |J_new - J_start| (COL_CONV)
+    int bad = 0;         // per lane: non-finite outputs of the fused step
 
     for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         double Jo[W][KPL], Hr[W][KPL], Hz[W][KPL];
@@ -538,6 +547,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                     if (k >= 1 && k <= nn - 2) jn = sB[(size_t)w * S + (k - 1)];
                     else if (!FUSED) jn = a.J[idx];
                 }
+                // the floor below would hide a NaN in J (max(NaN, Jeps) = Jeps): test first
+                if (FUSED) bad += !(__builtin_isfinite(jn) && __builtin_isfinite(hr) && __builtin_isfinite(hz));
                 if (HCORR) {
                     double Jk = dmax(jn, kJeps);
                     double inv_Jold = 1.0 / dmax(Jo[w][kk], kJeps);
@@ -556,6 +567,7 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                 if (!FUSED) { a.J_old[idx] = jn; a.Hr_old[idx] = hr; a.Hz_old[idx] = hz; }
                 if (FUSED) {
                     const double dk = a.dnu[k];
+                    if (CONV) conv += dk * fabs(jn - a.J_old[idx]) * dV;
                     e1[kk] += dk * dmax(jn - kJeps, 0.0) * dV;
                     e10[kk] += dk * dmax(jn - 10 * kJeps, 0.0) * dV;
                     // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
@@ -575,12 +587,16 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
             const int k = lane + 64 * kk;
             if (k < nn) { prow[k] = e1[kk]; prow[nn + k] = e10[kk]; }
         }
+        conv = wave_sum(conv);
+        const double nbad = wave_sum((double)bad);
         if (lane == 0) {
-            prow[2 * nn + 0] = acc_mf;
-            prow[2 * nn + 1] = acc_L;
-            prow[2 * nn + 2] = acc_net;
-            prow[2 * nn + 3] = acc_out;
-            prow[2 * nn + 4] = acc_esc;
+            prow[2 * nn + SUM_MF] = acc_mf;
+            prow[2 * nn + SUM_L] = acc_L;
+            prow[2 * nn + SUM_NET] = acc_net;
+            prow[2 * nn + SUM_OUT] = acc_out;
+            prow[2 * nn + SUM_ESC] = acc_esc;
+            prow[2 * nn + SUM_CONV] = conv;
+            prow[2 * nn + SUM_BAD] = nbad;
         }
     }
 }
@@ -590,12 +606,15 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
 // Compute_f_esc_Lyα_Cylindrical (:969-993) and FindActiveBins for the next step.
 __global__ void k_step_final(const double* __restrict__ red, int nn, int kcore_min, int kcore_max,
                              int bc_z_bot, int* __restrict__ krange, double* __restrict__ scalars,
-                             double* __restrict__ history)
+                             double* __restrict__ history, int* __restrict__ err_flag, StepCtl* __restrict__ ctl,
+                             double tol, int conv)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ctl && ctl->halt) return;
     const int kmin_used = krange[0], kmax_used = krange[1];
-    double MF = red[2 * nn + 0], L = red[2 * nn + 1], net = red[2 * nn + 2], out = red[2 * nn + 3];
-    double esc = red[2 * nn + 4];
+    double MF = red[2 * nn + SUM_MF], L = red[2 * nn + SUM_L], net = red[2 * nn + SUM_NET];
+    double out = red[2 * nn + SUM_OUT];
+    double esc = red[2 * nn + SUM_ESC];
     if (L > 0.0) {
         double s = 1.0 / (L / kC);
         MF = MF * s; net = net * s; out = out * s;
@@ -608,11 +627,24 @@ __global__ void k_step_final(const double* __restrict__ red, int nn, int kcore_m
         double Lsim = (bc_z_bot == BC_REFLECT) ? 0.5 * Ltot : Ltot;
         fesc = (Lsim > 0.0) ? esc / Lsim : 0.0;
     }
-    double v[8] = {MF, net, out, Ltot, fesc, (double)kmin_used, (double)kmax_used, 0.0};
+    // ΔJ/J = Σ Δν ΔV |J_new - J_start| / E_tot, E_tot = Σ Δν ΔV max(J_new - Jeps, 0) (the
+    // energy FindActiveBins measures, src/UpdateLyaJandH.jl:60-66)
+    double rel = 0.0;
+    if (conv) {
+        double Etot = 0.0;
+        for (int k = 0; k < nn; ++k) Etot += red[k];
+        rel = (Etot > 0.0) ? red[2 * nn + SUM_CONV] / Etot : 0.0;
+    }
+    if (red[2 * nn + SUM_BAD] != 0.0) atomicOr(err_flag, (int)ERR_NONFINITE);
+    double v[8] = {MF, net, out, Ltot, fesc, (double)kmin_used, (double)kmax_used, rel};
     for (int m = 0; m < 8; ++m) scalars[m] = v[m];
     if (history)
         for (int m = 0; m < 8; ++m) history[m] = v[m];
     active_range(red, nn, kcore_min, kcore_max, krange);
+    if (ctl) {
+        ctl->steps += 1;
+        if (tol > 0.0 && conv && rel < tol) ctl->halt = 1;
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -761,9 +793,10 @@ void launch_active_partial(hipStream_t s, const double* J, const double* dnu, co
                        pstride);
 }
 
-void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out)
+void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride, int ncols, double* out,
+                        const int* halt)
 {
-    hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(256), 0, s, part, nrows, stride, out);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(256), 0, s, part, nrows, stride, out, halt);
 }
 
 void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange)
@@ -794,22 +827,8 @@ int column_width() { return kColumnW; }
 // cloud: 1.15 ms against 0.81 ms for the kernel).
 int column_blocks(int nn, int ngroups)
 {
-    const size_t lds = 4 * (size_t)col_stride(nn) * kColumnW * sizeof(double);
-    int per_cu = 0;
-    hipError_t e;
-    switch ((nn + 63) / 64) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_column<kColumnW, 1>, 64, lds); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_column<kColumnW, 2>, 64, lds); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_column<kColumnW, 3>, 64, lds); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_column<kColumnW, 4>, 64, lds); break;
-    }
-    if (e != hipSuccess || per_cu < 1) per_cu = 8;
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                hipSuccess || cus < 1)
-        cus = 256;
-    const long nb = (long)per_cu * cus;
-    return (int)std::max<long>(1, std::min<long>(ngroups, nb));
+    (void)nn;
+    return std::max(1, std::min(ngroups, kColumnBlocks));
 }
 
 void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks)
@@ -824,10 +843,11 @@ void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks)
 }
 
 void launch_step_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int bc_z_bot,
-                       int* krange, double* scalars, double* history)
+                       int* krange, double* scalars, double* history, int* err_flag, StepCtl* ctl,
+                       double tol, int conv)
 {
     hipLaunchKernelGGL(k_step_final, dim3(1), dim3(64), 0, s, red, nn, kcmin, kcmax, bc_z_bot, krange, scalars,
-                       history);
+                       history, err_flag, ctl, tol, conv);
 }
 
 void launch_mf_partial(hipStream_t s, const MfArgs& a, int nblocks)
@@ -968,6 +988,313 @@ __global__ __launch_bounds__(64) void k_refresh(RefreshArgs a)
 void launch_refresh(hipStream_t s, const RefreshArgs& a)
 {
     hipLaunchKernelGGL(k_refresh, dim3(a.ncell), dim3(64), (size_t)(a.nn + 1) * sizeof(double), s, a);
+}
+
+}  // namespace m1
+
+namespace m1 {
+
+// ------------------------------------------------------------------------
+// RHD time-step control: one value per cell, reduced by min (CFL) or max (RSL).
+//   mode 0, GetRHDCFLTimeStep (src/AdvectJandH.jl:29-68): 0.5 min(Δr, ΔZ) / (c̃ + |u|)
+//   mode 1, FindRSL_LyαRT (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:655-705):
+//           ϵ_RSL max(1, (aτ)^{1/3} f_abs) max(|u|, c_s)
+// ------------------------------------------------------------------------
+__device__ double rhd_cell_value(const RhdScanArgs& a, int c)
+{
+    const int i = c % a.nr, j = c / a.nr;
+    const double dr = a.r[i + 1] - a.r[i];
+    const double dZ = a.Z[j + 1] - a.Z[j];
+    const double u = hypot(a.u_r[c], a.u_z[c]);
+    if (a.mode == 0) return 0.5 * dmin(dr, dZ) / (a.c_tilde + u);
+    const double c_s = sqrt(a.gamma_ad[c] * a.P[c] / a.rho[c]);
+    const double xHI = dmax(1 - a.xHII[c] - 2 * a.xH2[c], 0.0);
+    const double nHI = xHI * a.X_over_mH * a.rho[c];
+    const double T = a.T[c];
+    const double b = a.lc.b_prefact * sqrt(T);
+    const double alpha = nHI * (a.lc.sigma0_prefact / b);
+    const double dX = dmax(dr, dZ);
+    const double tcell = alpha * dX;
+    const double tcabs = (a.kS * a.rho_d[3 * c] + a.kG * a.rho_d[3 * c + 1] + a.kP * a.rho_d[3 * c + 2]) * dX;
+    const double at = (a.lc.a_prefact / b) * tcell;
+    const double at13 = pow(at, 1.0 / 3.0);
+    const double fabs_ = 1 / (1 + at13 * tcabs + 0.6 * a.pd[c] * tcell);
+    return 4.0 * dmax(1, at13 * fabs_) * dmax(u, c_s);   // ϵ_RSL = 4 (PhotoThermoChemistry.jl:34)
+}
+
+__global__ __launch_bounds__(256) void k_rhd_scan(RhdScanArgs a, double* __restrict__ part)
+{
+    __shared__ double s[256];
+    const int n = a.nr * a.nz;
+    const bool mx = a.mode == 1;
+    double v = mx ? 0.0 : INFINITY;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
+        const double x = rhd_cell_value(a, c);
+        v = mx ? dmax(v, x) : dmin(v, x);
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = mx ? dmax(s[threadIdx.x], s[threadIdx.x + w])
+                                                 : dmin(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_rhd_scan_final(const double* __restrict__ part, int nb, int mode,
+                                                         double* __restrict__ out)
+{
+    __shared__ double s[256];
+    const bool mx = mode == 1;
+    double v = mx ? 0.0 : INFINITY;
+    for (int b = threadIdx.x; b < nb; b += 256) v = mx ? dmax(v, part[b]) : dmin(v, part[b]);
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = mx ? dmax(s[threadIdx.x], s[threadIdx.x + w])
+                                                 : dmin(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+void launch_rhd_scan(hipStream_t s, const RhdScanArgs& a, int nblocks, double* part, double* out)
+{
+    hipLaunchKernelGGL(k_rhd_scan, dim3(nblocks), dim3(256), 0, s, a, part);
+    hipLaunchKernelGGL(k_rhd_scan_final, dim3(1), dim3(256), 0, s, part, nblocks, a.mode, out);
+}
+
+// ------------------------------------------------------------------------
+// The Lyα block of AbsorptionStep (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:1577-1670):
+// one wavefront per cell, lanes over the bins.  Backward-Euler J (floored at Jeps) and H
+// over the cell's sub-step, the momentum each bin deposits split between gas and the
+// three dust species by flux opacity (SplitRadPressure, :1017-1048), the would-be e_Lyα and
+// the rejection rule (tol_Abs = 0.1, e_Lyα_floor = (4π/c) 1e-15: :1051-1055); a rejected
+// cell keeps its old J and H (RestoreLyαRejection, :980-990).
+// ------------------------------------------------------------------------
+template <int KPL>
+__global__ __launch_bounds__(64) void k_absorb(AbsorbArgs a)
+{
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (c >= a.ncell) return;
+    const int nn = a.nn;
+    const double ct = a.c_tilde, inv_ct = 1.0 / ct;
+    const double dts = a.dt[c];
+    const double ct_js = ct * a.js[c];
+    const double ac = a.alpha_c[c];
+    const double pd_a = a.pd[c] * a.alpha[c];
+    const double aS = a.rho_d ? a.kS * a.rho_d[3 * c] : 0.0;
+    const double aG = a.rho_d ? a.kG * a.rho_d[3 * c + 1] : 0.0;
+    const double aP = a.rho_d ? a.kP * a.rho_d[3 * c + 2] : 0.0;
+    const double dt_lya = inv_ct / (ac + pd_a);
+    const size_t base = (size_t)c * nn;
+    double Jn[KPL], Hrn[KPL], Hzn[KPL];
+    double sJ = 0.0, mr = 0.0, mz = 0.0, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < KPL; ++kk) {
+        const int k = lane + 64 * kk;
+        Jn[kk] = 0.0; Hrn[kk] = 0.0; Hzn[kk] = 0.0;
+        if (k >= nn) continue;
+        const size_t idx = base + k;
+        const double dnu = a.dnu[k];
+        const double dJdt = ct_js * a.phi[idx];
+        const double a_abs = ac + pd_a * a.Hv[idx];
+        const double Jo = a.J_old[idx], Hro = a.Hr_old[idx], Hzo = a.Hz_old[idx];
+        // AbsorptionUpdateJorH (src/ConstantsAndFunctions.jl:451-462)
+        Jn[kk] = dmax((Jo + dJdt * dts) / (1 + a_abs * ct * dts), kJeps);
+        const double aFk = a.aF[idx];
+        Hrn[kk] = (Hro + 0.0 * dts) / (1 + aFk * ct * dts);
+        Hzn[kk] = (Hzo + 0.0 * dts) / (1 + aFk * ct * dts);
+        const double w = dnu * (kFourPi / kC);
+        const double dr_k = w * (inv_ct * (Hro - Hrn[kk]));
+        const double dz_k = w * (inv_ct * (Hzo - Hzn[kk]));
+        mr += dr_k;
+        mz += dz_k;
+        const double a_gas = dmax(aFk - aS - aG - aP, 0.0);
+        const double a_tot = a_gas + aS + aG + aP;
+        double fg = 1.0, fS = 0.0, fG = 0.0, fP = 0.0;
+        if (a_tot > 0.0) {
+            const double inv = 1 / a_tot;
+            fg = a_gas * inv; fS = aS * inv; fG = aG * inv; fP = aP * inv;
+        }
+        g[0] += fg * dr_k; g[1] += fg * dz_k;
+        g[2] += fS * dr_k; g[3] += fS * dz_k;
+        g[4] += fG * dr_k; g[5] += fG * dz_k;
+        g[6] += fP * dr_k; g[7] += fP * dz_k;
+        sJ += dnu * Jn[kk];
+    }
+    sJ = wave_sum(sJ);
+    mr = wave_sum(mr);
+    mz = wave_sum(mz);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) g[m] = wave_sum(g[m]);
+    const double e_new = (kFourPi / kC) * sJ;
+    const double e_old = a.e_lya[c];
+    const double de = fabs(e_new - e_old) / (e_old + (kFourPi / kC) * 1e-15);
+    const bool rej = (de > 0.1 && dts > 0.1 * dt_lya) || !__builtin_isfinite(de);
+#pragma unroll
+    for (int kk = 0; kk < KPL; ++kk) {
+        const int k = lane + 64 * kk;
+        if (k >= nn) continue;
+        const size_t idx = base + k;
+        a.J[idx] = rej ? a.J_old[idx] : Jn[kk];
+        a.Hr[idx] = rej ? a.Hr_old[idx] : Hrn[kk];
+        a.Hz[idx] = rej ? a.Hz_old[idx] : Hzn[kk];
+    }
+    if (lane == 0) {
+        const double v[12] = {de, e_new, mr, mz, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7]};
+        a.redo[c] = rej ? 1 : 0;
+        for (int m = 0; m < 12; ++m) a.out[(size_t)m * a.ncell + c] = (rej && m > 0) ? 0.0 : v[m];
+    }
+}
+
+void launch_absorb(hipStream_t s, const AbsorbArgs& a)
+{
+    const int kpl = (a.nn + 63) / 64;
+    switch (kpl) {
+    case 1: hipLaunchKernelGGL(k_absorb<1>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_absorb<2>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_absorb<3>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_absorb<4>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    }
+}
+
+}  // namespace m1
+
+namespace m1 {
+
+// ------------------------------------------------------------------------
+// Lyα source terms of PrepareLyαSubCycling! (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2086-2155):
+// the frequency-integrated emissivity from recombinations (src/LymanAlphaEmission.jl:105-124) and
+// collisional excitation (:143-173), the destruction probability (src/LymanAlphaDestruction.jl:43-97)
+// and the dust + gas continuum absorption and flux opacities, one thread per cell.
+// ------------------------------------------------------------------------
+// Storey & Hummer (1995) probability of Lyα emission per recombination (src/LymanAlphaEmission.jl:30-60),
+// T = 5e2 .. 3e4 K (10 nodes) by n_e = 1e2 .. 1e14 cm^-3 (Case B, 13 nodes) or 1e2 .. 1e10 (Case A, 9 nodes).
+__constant__ double SH95_T[10] = {5e2, 1e3, 3e3, 5e3, 7.5e3, 1e4, 1.25e4, 1.5e4, 2e4, 3e4};
+__constant__ double SH95_B[10][13] = {
+    {0.80793637, 0.90215197, 0.98157401, 0.9970534, 0.99933853, 0.99934845, 0.99967896, 0.99948547, 0.99931146, 0.99951894, 0.9997498, 0.99864818, 0.99372263},
+    {0.78439303, 0.87743841, 0.97423464, 0.99680114, 0.99960243, 0.99967816, 0.9997671, 0.99939133, 0.99922946, 0.99960973, 0.99937386, 0.99906812, 0.9954834},
+    {0.74159266, 0.82809599, 0.95734434, 0.99359891, 0.99892162, 0.99927343, 0.99979471, 0.99938909, 0.99899055, 0.99925979, 0.99957955, 0.99923525, 0.99703571},
+    {0.71946081, 0.80133228, 0.94581237, 0.99246182, 0.99848958, 0.99940329, 0.99949246, 0.99952048, 0.99981447, 0.99956726, 0.9993515, 0.99952024, 0.99692499},
+    {0.70117195, 0.7788453, 0.93441344, 0.99075058, 0.99831301, 0.99960223, 0.99940949, 0.99946884, 0.99948297, 0.99947146, 0.99965311, 0.99918254, 0.99690311},
+    {0.68808416, 0.76220739, 0.92503048, 0.9891268, 0.99828523, 0.9994074, 0.99944339, 0.99952601, 0.99932358, 0.99927528, 0.99938987, 0.99904232, 0.9949615},
+    {0.6775581, 0.74868826, 0.9176273, 0.98784252, 0.99835238, 0.99903921, 0.99977064, 0.99954296, 0.99930529, 0.99954743, 0.99944997, 0.99874439, 0.9916493},
+    {0.66862468, 0.73785038, 0.91033402, 0.98701405, 0.99817046, 0.99892625, 0.99953078, 0.99960556, 0.9993362, 0.99941558, 0.99944365, 0.99828039, 0.98817287},
+    {0.65534642, 0.72080463, 0.89823705, 0.98497224, 0.99798854, 0.9988133, 0.99929092, 0.99909416, 0.99933199, 0.9995549, 0.99924446, 0.99764512, 0.97960152},
+    {0.63566376, 0.69600394, 0.87952243, 0.98121463, 0.99716593, 0.99896144, 0.99905958, 0.9996252, 0.99953649, 0.99931002, 0.99872497, 0.99561256, 0.96391965}};
+__constant__ double SH95_A[10][9] = {
+    {0.58042473, 0.59595911, 0.60335363, 0.58990186, 0.56623927, 0.53426043, 0.492528, 0.44534414, 0.40955195},
+    {0.54712958, 0.56465163, 0.57999443, 0.57241623, 0.5541246, 0.5276139, 0.49052298, 0.44606687, 0.4101595},
+    {0.48320245, 0.50151448, 0.52818381, 0.53008608, 0.52008701, 0.50298698, 0.47625175, 0.43924427, 0.40696841},
+    {0.44933274, 0.46714191, 0.49852558, 0.50475148, 0.49815515, 0.48505756, 0.46319941, 0.43069186, 0.40125666},
+    {0.42087234, 0.43810743, 0.47280049, 0.48268151, 0.47821493, 0.46794659, 0.4497736, 0.42097532, 0.39442497},
+    {0.40023971, 0.41675887, 0.45373431, 0.46591319, 0.46300415, 0.45454038, 0.43876097, 0.41249415, 0.38810578},
+    {0.3840314, 0.40007514, 0.43832241, 0.45257107, 0.45064954, 0.4434932, 0.42939783, 0.40511911, 0.38230403},
+    {0.37046215, 0.38617473, 0.42546609, 0.44143608, 0.44049428, 0.43394168, 0.42127472, 0.39854551, 0.37684759},
+    {0.34937867, 0.36397106, 0.40484801, 0.42349201, 0.42367822, 0.41860328, 0.40761256, 0.38699493, 0.36739625},
+    {0.31960139, 0.33302781, 0.37528203, 0.39805701, 0.39984039, 0.39602958, 0.38718651, 0.36963656, 0.3518518}};
+
+
+__device__ double sh95(double T, double ne, bool caseB)
+{
+    const int nn = caseB ? 13 : 9;
+    const double* f = caseB ? &SH95_B[0][0] : &SH95_A[0][0];
+    const double x = dmin(dmax(log10(T), log10(SH95_T[0])), log10(SH95_T[9]));
+    const double y = dmin(dmax(log10(ne), 2.0), (double)(nn + 1));
+    int i = 0, j = 0;
+    while (i < 8 && x > log10(SH95_T[i + 1])) ++i;
+    while (j < nn - 2 && y > (double)(j + 3)) ++j;
+    const double xi = log10(SH95_T[i]), xi1 = log10(SH95_T[i + 1]);
+    const double t = (x - xi) / (xi1 - xi);
+    const double u = (y - (double)(j + 2)) / ((double)(j + 3) - (double)(j + 2));
+    const double f0 = f[i * nn + j] + u * (f[i * nn + j + 1] - f[i * nn + j]);
+    const double f1 = f[(i + 1) * nn + j] + u * (f[(i + 1) * nn + j + 1] - f[(i + 1) * nn + j]);
+    return f0 + t * (f1 - f0);
+}
+
+__device__ double krec_hii(double T, bool caseB)
+{
+    T = dmin(dmax(T, 1.0), 1e9);
+    const double lam = 315614.0 / T;
+    if (caseB) return 2.753e-14 * pow(lam, 1.5) / pow(1 + pow(lam / 2.74, 0.407), 2.242);
+    return 1.269e-13 * pow(lam, 1.503) / pow(1 + pow(lam / 0.522, 0.47), 1.923);
+}
+
+__global__ __launch_bounds__(256) void k_sources(SourceArgs a)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.ncell) return;
+    const int n = a.ncell;
+    const double T = a.chem[c], nHI = a.chem[n + c], nH2 = a.chem[2 * n + c], nHII = a.chem[3 * n + c];
+    const double nHeI = a.chem[4 * n + c], ne = a.chem[5 * n + c], nOH = a.chem[6 * n + c], nCH = a.chem[7 * n + c];
+    constexpr double kE = (6.626069573e-27 * 2.466068e15) * (1.0 / kFourPi);   // E_Lyα / 4π
+    // recombinations (Case B with the on-the-spot approximation, else Case A)
+    const double js_rec = sh95(T, ne, a.ots) * krec_hii(T, a.ots) * nHII * ne * kE;
+    // collisional excitation
+    constexpr double T_ryd = (6.626069573e-27 * 3.289842e15) / 1.380648813e-16;
+    constexpr double T12 = (1 - 1.0 / 4.0) * T_ryd, T13 = (1 - 1.0 / 9.0) * T_ryd;
+    constexpr double T14 = (1 - 1.0 / 16.0) * T_ryd, T15 = (1 - 1.0 / 25.0) * T_ryd;
+    const double Tc = dmax(T, 1.0);
+    const double inv_T = 1.0 / Tc;
+    const double T6 = dmin(Tc, 3e5) * 1e-6;
+    const double T6s = T6 * T6, T6c = T6s * T6;
+    const double Y12 = 0.267486 + 1.57257 * T6 - 6.44026 * T6s + 11.5401 * T6c;
+    const double Y13 = 0.0940821 + 2.90748 * T6 - 6.69528 * T6s + 7.86419 * T6c;
+    const double Y14 = 0.0415028 + 1.13673 * T6 - 3.97454 * T6s + 6.05941 * T6c;
+    const double Y15 = 0.032277 + 0.40856 * T6 - 1.34604 * T6s + 2.07558 * T6c;
+    const double q = (8.63e-6 / (2 * sqrt(Tc))) *
+                     (Y12 * exp(-T12 * inv_T) + Y13 * exp(-T13 * inv_T) + Y14 * exp(-T14 * inv_T) +
+                      Y15 * exp(-T15 * inv_T));
+    const double js = js_rec + q * nHI * ne * kE;
+    // destruction probability
+    double pd = 0.0;
+    if (a.destruction) {
+        const double Tp = dmin(dmax(T, 1.0), 1e5);
+        const double T100 = Tp / 100.0;
+        const double lT = log10(Tp), lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT3 * lT;
+        const double G2p2s = 3.1e-6 * (1 + a.z), G2s2p = 3 * G2p2s;
+        const double r25 = 2.5 / Tp;
+        const double C_e = 5e-5 * exp(-(r25 * r25)) * ne / (1 + pow(T100, 0.45));
+        const double C_HII = (6.98e-3 * (1 + 0.933 * lT) * exp(-pow(601 / Tp, 0.406)) / pow(Tp, 0.5) +
+                              pow(10.0, -7.69 + 1.21 * lT + 0.455 * lT2 - 0.199 * lT3 + 0.0182 * lT4)) * nHII;
+        const double C_HeI = 2.6e-9 * pow(T100, 0.315) * nHeI;
+        const double C_H2 = 1.7e-9 * pow(T100, 0.3) * nH2;
+        const double den = 8.9 * pow(Tp, -0.55) + 0.078 * pow(Tp, 0.35);
+        const double C_HI = (4.1e-11 * exp(-pow(1.6 / Tp, 3.0 / 4.0) - 0.15 * Tp) +
+                             (1.6e-10 + 1.57e-12 * pow(Tp, 1.1)) / (den * den)) * nHI;
+        const double C2s2p = C_e + C_HII + C_HeI + C_H2 + C_HI;
+        const double C2p2s = (1.0 / 3.0) * C2s2p;
+        const double R2p2s = (2.0 / 3.0) * 8.78e-7 + C2p2s + G2p2s;
+        const double R2s2p = 1.597e-9 + C2s2p + G2s2p;
+        const double pd_HI = R2p2s / ((6.265e8 / 8.2206) * (8.2206 + R2s2p) + R2p2s);
+        double pd_H2 = 0.0;
+        if (nHI > 1e-20)
+            pd_H2 = dmin((33.0 * exp(-14540.0 / Tp) + 13.0 * exp(-15564.0 / Tp)) /
+                             (46.0 + 1e3 * exp(-pow(1e3 / Tp, 1.5))),
+                         1.3e-3) * (nH2 / nHI);
+        pd = dmin(pd_HI + pd_H2, 1.0);
+    }
+    // continuum opacities: dust below 1.1 ρd_floor is ignored
+    constexpr double rd_ignore = 1.1 * (1e-13 * (1.0080 * 1.660539069e-24));
+    double rd[3] = {0.0, 0.0, 0.0};
+    if (a.rho_d)
+        for (int s = 0; s < 3; ++s) rd[s] = (a.rho_d[3 * c + s] > rd_ignore) ? a.rho_d[3 * c + s] : 0.0;
+    const double ac_gas = nH2 * 2.1e-25 + nOH * 4.07e-18 + nCH * 0.05e-18;
+    const double acF_gas = ac_gas + ne * 6.6524587321e-25;
+    a.js[c] = js;
+    a.alpha_c[c] = a.kappa[0] * rd[0] + a.kappa[1] * rd[1] + a.kappa[2] * rd[2] + ac_gas;
+    a.acF[c] = a.kappa[3] * rd[0] + a.kappa[4] * rd[1] + a.kappa[5] * rd[2] + acF_gas;
+    a.pd[c] = pd;
+    a.T[c] = T;
+    a.nHI[c] = nHI;
+}
+
+void launch_sources(hipStream_t s, const SourceArgs& a)
+{
+    hipLaunchKernelGGL(k_sources, dim3((a.ncell + 255) / 256), dim3(256), 0, s, a);
 }
 
 }  // namespace m1

@@ -19,7 +19,8 @@ module M1LyaGPU
 export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
        Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!,
-       set_voigt_lut!, prepare_subcycling!, set_adjoint_mode!
+       set_voigt_lut!, prepare_subcycling!, set_adjoint_mode!, run_until_steady!, GetRHDCFLTimeStep,
+       FindRSL_LyαRT, absorption_substep!
 
 const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
 
@@ -184,6 +185,18 @@ function Update_f_Lyα_MF_L_Lyα!(f_Lyα_r, f_Lyα_Z, Δρu_Lyα_r, Δρu_Lyα_Z
     return out[1], out[2], out[3], out[4]
 end
 
+# --- src/ConstantsAndFunctions.jl:691-704, the variant with gravity (f_grav_r/Z [N_r, N_Z]) ---
+function Update_f_Lyα_MF_L_Lyα!(f_Lyα_r, f_Lyα_Z, Δρu_Lyα_r, Δρu_Lyα_Z, ∫dν_js, ΔV, rCoM, ZCoM,
+                                f_grav_r::AbstractMatrix{Float64}, f_grav_Z::AbstractMatrix{Float64},
+                                Δt::Float64, Z_center::Float64, BC_Z_bot::Symbol=:Reflect)
+    ctx = ctxof(only(k for (k, _) in CONTEXTS if k.f_Lyα_r === f_Lyα_r))
+    out = zeros(5)
+    GC.@preserve f_grav_r f_grav_Z check(ccall((:m1lya_update_f_mf_l_gravity, libm1lya), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Float64, Cint, Ptr{Float64}),
+        ctx, Δt, f_grav_r, f_grav_Z, Z_center, bc(BC_Z_bot), out), ctx)
+    return out[1], out[2], out[3], out[4], out[5]
+end
+
 # --- src/ConstantsAndFunctions.jl:969-976 ---
 function Compute_f_esc_Lyα_Cylindrical(H_r_Lya, H_Z_Lya, A_r, A_Z, Δν, L_Lyα_allspace::Float64,
                                        BC_Z_bot::Symbol=:Reflect)
@@ -335,6 +348,69 @@ function prepare_subcycling!(Lyα, T::Matrix{Float64}, nHI::Matrix{Float64}; αc
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
         ctx, T, nHI, optr(αc), optr(αc_F), optr(js), optr(pd), Cint(all_bins)), ctx)
     return nothing
+end
+
+# --- steady-state option of the RT-only static cloud (the reference runs a fixed horizon,
+#     src/StaticCloudTest.jl:317-322): fused steps until ΔJ/J < tol ---
+"""Runs fused steps until ΔJ/J = Σ Δν ΔV |J_new - J_start| / Σ Δν ΔV max(J_new - Jeps, 0) < tol or
+max_steps; returns the [8, steps] history (row 8 is ΔJ/J)."""
+function run_until_steady!(Lyα, max_steps::Integer, tol::Float64, c̃::Float64, Δt::Float64, Z_center::Float64;
+                           BC_r_right::Symbol=:Outflow, BC_Z_bot::Symbol=:Reflect, BC_Z_top::Symbol=:Outflow)
+    ctx = ctxof(Lyα)
+    hist = zeros(8, max(max_steps, 1))
+    done = Ref{Cint}(0)
+    check(ccall((:m1lya_run_until_steady, libm1lya), Cint,
+                (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Float64, Cint, Cint, Cint, Ptr{Float64}, Ref{Cint}),
+                ctx, max_steps, tol, c̃, Δt, Z_center, bc(BC_r_right), bc(BC_Z_bot), bc(BC_Z_top), hist, done), ctx)
+    return hist[:, 1:done[]]
+end
+
+# --- src/AdvectJandH.jl:29-68: the CFL step as a device reduction on the context of this grid ---
+function GetRHDCFLTimeStep(r::AbstractVector{Float64}, Z::AbstractVector{Float64}, c̃::Float64,
+                           u_r::AbstractArray{Float64,2}, u_Z::AbstractArray{Float64,2}, CFL::Float64)
+    ctx = only(c for (k, c) in CONTEXTS if size(k.α_Lyα) == size(u_r))
+    dt = Ref{Float64}(0.0)
+    GC.@preserve u_r u_Z check(ccall((:m1lya_cfl_time_step, libm1lya), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Float64, Ref{Float64}), ctx, c̃, u_r, u_Z, CFL, dt), ctx)
+    return dt[]
+end
+
+struct CRslInputs          # m1lya_rsl_inputs
+    u_r::Ptr{Float64}; u_z::Ptr{Float64}; gamma_ad::Ptr{Float64}; P::Ptr{Float64}; rho::Ptr{Float64}
+    rho_d::Ptr{Float64}; xHII::Ptr{Float64}; xH2::Ptr{Float64}; T::Ptr{Float64}
+    kappa_abs::NTuple{3,Float64}; lc::CLineConstants
+end
+
+# --- src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:616-722 ---
+function FindRSL_LyαRT(c̃_Lyα::Float64, c̃_min::Float64, Lyα, Hydro, Chem, Dust, Geom; lc::CLineConstants)
+    isnothing(Lyα) && return 0.0
+    ctx = ctxof(Lyα)
+    u_r, u_Z, P, ρ, ρd, γad, T = Hydro.u_X, Hydro.u_Y, Hydro.P, Hydro.ρ, Hydro.ρd, Hydro.γad, Hydro.T
+    xHII, xH2 = Chem.xHII, Chem.xH2
+    out = Ref{Float64}(0.0)
+    GC.@preserve u_r u_Z P ρ ρd γad T xHII xH2 begin
+        ri = CRslInputs(pointer(u_r), pointer(u_Z), pointer(γad), pointer(P), pointer(ρ), pointer(ρd),
+                        pointer(xHII), pointer(xH2), pointer(T),
+                        (Dust.κd_abs_Lyα_Sil, Dust.κd_abs_Lyα_C, Dust.κd_abs_Lyα_PAH), lc)
+        check(ccall((:m1lya_find_rsl, libm1lya), Cint, (Ptr{Cvoid}, Float64, Float64, Ref{CRslInputs}, Ref{Float64}),
+                    ctx, c̃_Lyα, c̃_min, ri, out), ctx)
+    end
+    return out[]
+end
+
+# --- the Lyα block of AbsorptionStep (PhotoThermoChemistry.jl:1577-1670) for every cell at once ---
+"""Sub-step Δt_cells [N_r, N_Z] of every cell: updates J, H on the device from the *_old arrays,
+rejects (and restores) cells whose e_Lyα changes by more than tol_Abs; returns (redo::Matrix{Cint},
+out::Array{Float64,3}) with out[:, :, m] = Δe, e_Lyα_new, Δρu_r/Z_Lyα, then r/Z of gas, Sil, C, PAH."""
+function absorption_substep!(Lyα, Δt_cells::Matrix{Float64}, c̃_Lyα::Float64, ρd, Dust)
+    ctx = ctxof(Lyα)
+    redo = zeros(Cint, size(Δt_cells))
+    out = zeros(size(Δt_cells)..., 12)
+    κF = [Dust.κd_F_Lyα_Sil, Dust.κd_F_Lyα_C, Dust.κd_F_Lyα_PAH]
+    GC.@preserve Δt_cells ρd κF check(ccall((:m1lya_absorption_substep, libm1lya), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Cint}, Ptr{Float64}),
+        ctx, Δt_cells, 0.0, c̃_Lyα, ρd, κF, redo, out), ctx)
+    return redo, out
 end
 
 end # module

@@ -64,7 +64,7 @@ class M1LyaContext:
         del arrs
         for key in ("Hvoigt", "phi", "alpha_F", "alpha", "alpha_c", "pd", "js", "dnuD", "T",
                     "dur_dr", "dur_dz", "duz_dr", "duz_dz", "ur_over_r", "ur_star", "uz_star",
-                    "drhou_r", "drhou_z"):
+                    "drhou_r", "drhou_z", "e_lya"):
             if key in lya.arrays:
                 self.upload(key, lya[key])
         if upload_state:
@@ -259,6 +259,19 @@ def run_steps(ctx: M1LyaContext, n_steps: int, c_tilde: float, dt: float, z_cent
                                        int(bc_r_right), int(bc_z_bot), int(bc_z_top),
                                        ptr(hist) if hist is not None else None))
     return hist
+
+
+def run_until_steady(ctx: M1LyaContext, max_steps: int, tol: float, c_tilde: float, dt: float, z_center: float,
+                     bc_r_right=OUTFLOW, bc_z_bot=REFLECT, bc_z_top=OUTFLOW) -> np.ndarray:
+    """Fused steps until ΔJ/J = Σ Δν ΔV |J_new - J_start| / Σ Δν ΔV max(J_new - Jeps, 0) < tol, at
+    most max_steps.  The reference runs a fixed horizon (src/StaticCloudTest.jl:317-322); this is
+    the steady-state option.  Returns the (steps run, 8) history; column 7 holds each step's ΔJ/J."""
+    hist = np.empty((max(int(max_steps), 1), 8))
+    done = C.c_int(0)
+    ctx._check(ctx.lib.m1lya_run_until_steady(ctx.handle, int(max_steps), float(tol), float(c_tilde), float(dt),
+                                              float(z_center), int(bc_r_right), int(bc_z_bot), int(bc_z_top),
+                                              ptr(hist), C.byref(done)))
+    return hist[:done.value].copy()
 
 
 def selftest_math(op: int, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:

@@ -188,28 +188,40 @@ def random_state(shape: tuple, seed: int = DEFAULT_SEED, scale: float = 1.0, jep
     return J, Hr, Hz
 
 
-def developed_state(cloud: StaticCloud, seed: int = DEFAULT_SEED, floor: float = 1e-6, noise: float = 0.05):
+def developed_state(cloud: StaticCloud, seed: int = DEFAULT_SEED, floor: float = 1e-6, noise: float = 0.05,
+                    f_esc0: float = 0.5):
     """A seeded, smooth 'developed' radiation field for benchmarking (not a solution of the
     equations): the static-sphere emergent-spectrum shape (x²/aτ) sech²(√(π³/54)|x|³/aτ)
     (src/ConstantsAndFunctions.jl:1018-1025) at every cell, scaled by 1/(1 + (R/Δr)²), with a
     relative floor so that every frequency bin is active, and a radial flux H = f J R̂ with
-    f growing from 0.05 at the centre to 0.6 at the box edge."""
+    f growing from 0.05 at the centre to 0.6 at the box edge.  The amplitude is tied to the
+    source: the flux leaving the box carries f_esc0 of the luminosity L = 4π Σ ΔV ∫js
+    (Compute_f_esc_Lyα_Cylindrical, src/ConstantsAndFunctions.jl:914-993)."""
     lya, g = cloud.lya, cloud.geometry
     rng = np.random.Generator(np.random.PCG64(seed))
     T0 = cloud.params["T0"]
     a_tau = compute_a(T0) * cloud.params["tau_cl"]
     x = (lya["nuC"] - K.nu_Lya) / float(np.asarray(lya["dnuD"]).reshape(-1)[0])
-    prof = (x * x / a_tau) / np.cosh(np.sqrt(np.pi ** 3 / 54.0) * np.abs(x) ** 3 / a_tau) ** 2
+    arg = np.minimum(np.sqrt(np.pi ** 3 / 54.0) * np.abs(x) ** 3 / a_tau, 300.0)
+    prof = (x * x / a_tau) / np.cosh(arg) ** 2
     prof = prof / prof.max() + floor
     rc = g.rCoM[None, :]
     zc = g.ZCoM[:, None] - cloud.Zc
     R = np.sqrt(rc * rc + zc * zc)
     dr = g.r[1] - g.r[0]
-    amp = 1e-3 / (1.0 + (R / dr) ** 2)
+    amp = 1.0 / (1.0 + (R / dr) ** 2)
     J = amp[:, :, None] * prof[None, None, :] * np.exp(noise * rng.normal(size=(lya.n_z, lya.n_r, lya.n_nu)))
     f = 0.05 + 0.55 * (R / R.max())
     Hr = (f * rc / R)[:, :, None] * J
     Hz = (f * zc / R)[:, :, None] * J
+    dnu = lya["dnu"]
+    esc = K.Four_pi * (np.sum(g.A_r[:, -1, None] * np.maximum(Hr[:, -1, :], 0.0) * dnu)
+                       + np.sum(g.A_Z[-1, :, None] * np.maximum(Hz[-1, :, :], 0.0) * dnu))
+    if cloud.bc_z_bot == OUTFLOW:
+        esc += K.Four_pi * np.sum(g.A_Z[0, :, None] * np.maximum(-Hz[0, :, :], 0.0) * dnu)
+    L_sim = K.Four_pi * float(np.sum(g.dV * np.asarray(lya["js"])))
+    s = f_esc0 * L_sim / esc if esc > 0.0 and L_sim > 0.0 else 1e-3
+    J, Hr, Hz = J * s, Hr * s, Hz * s
     for k, v in (("J", J), ("H_r", Hr), ("H_z", Hz), ("J_old", J), ("H_r_old", Hr), ("H_z_old", Hz)):
         lya[k] = np.ascontiguousarray(v)
     return cloud

@@ -1293,3 +1293,280 @@ double orc_cone_factor(double mu, double mucone, double taucone, double taucl,
     if (smooth) return fc + 0.5 * (1.0 - fc) * (1.0 - tanh((mu - mucone) / dmu));
     return (mu > mucone) ? fc : 1.0;
 }
+
+
+/* ------------------------------------------------------------------------
+ * RHD time-step control and the Lyα absorption sub-step
+ * ------------------------------------------------------------------------ */
+#define X_HE      0.247
+#define D_H_RATIO 2.53e-5
+#define M_D       (2.014 * AMU)
+#define EPS_RSL   4.0
+#define TOL_ABS   0.1
+#define J_FLOOR_ABS 1e-15
+
+/* FindRSL_LyαRT, src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:616-722 (the Lyα
+ * branch; X_over_mH from src/ConstantsAndFunctions.jl:124-128,167, ϵ_RSL from
+ * PhotoThermoChemistry.jl:34).  rho_d is [n_z][n_r][3] (the reference's ρd[1:3,i,j]),
+ * kappa_abs the three κd_abs_Lyα of silicate, graphite and PAH dust. */
+double orc_find_rsl(double c_old, double c_min, const orc_geom *G, const double *u_r, const double *u_z,
+                    const double *gamma_ad, const double *P, const double *rho, const double *rho_d,
+                    const double *xHII, const double *xH2, const double *T, const double *pd,
+                    const double *kappa_abs)
+{
+    const double X_D = D_H_RATIO * (1 - X_HE) * M_D / M_H;
+    const double X_H = 1 - X_HE - X_D;
+    const double X_over_mH = X_H / M_H;
+    double cnew = 0.0;
+    for (int j = 0; j < G->n_z; ++j) {
+        double local = cnew;
+        const double dZ = G->z[j + 1] - G->z[j];
+        for (int i = 0; i < G->n_r; ++i) {
+            const int c = j * G->n_r + i;
+            const double dr = G->r[i + 1] - G->r[i];
+            const double c_s = sqrt(gamma_ad[c] * P[c] / rho[c]);
+            const double xHI = jmax(1 - xHII[c] - 2 * xH2[c], 0.0);
+            const double nHI = xHI * X_over_mH * rho[c];
+            const double alpha = nHI * orc_compute_sigma0(T[c]);
+            const double dX = jmax(dr, dZ);
+            const double tcell = alpha * dX;
+            const double tcabs = (kappa_abs[0] * rho_d[3 * c] + kappa_abs[1] * rho_d[3 * c + 1] +
+                                  kappa_abs[2] * rho_d[3 * c + 2]) * dX;
+            const double at = orc_compute_a(T[c]) * tcell;
+            const double at13 = pow(at, 1.0 / 3.0);
+            const double fabs_ = 1 / (1 + at13 * tcabs + 0.6 * pd[c] * tcell);
+            const double u = hypot(u_r[c], u_z[c]);
+            const double cmin_cell = EPS_RSL * jmax(1, at13 * fabs_) * jmax(u, c_s);
+            local = jmax(local, cmin_cell);
+        }
+        cnew = jmax(cnew, local);
+    }
+    cnew = jmin(jmax(cnew, c_min), C_LIGHT);
+    if (cnew < c_old) cnew = jmax(cnew, 0.75 * c_old);
+    if (cnew > c_old) cnew = jmin(cnew, 4.0 * c_old);
+    return cnew;
+}
+
+/* The Lyα block of AbsorptionStep (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:1577-1670)
+ * for every cell, with SplitRadPressure (:1017-1048) per bin, the rejection rule (tol_Abs,
+ * e_Lyα_floor: :1051-1055) and RestoreLyαRejection (:980-990) on rejected cells.
+ * dt [n_z][n_r] is each cell's sub-step; rho_d [n_z][n_r][3] may be NULL (no dust);
+ * kappa_F the three κd_F_Lyα.  Reads J_old, H_old, e_lya; writes J, H (the old values on a
+ * rejected cell).  redo [n_z][n_r]; out [12][n_z][n_r] = {Δe_Lyα, e_Lyα_new, Δρu_r/Z_Lyα,
+ * Δρu_r/Z of gas, silicate, graphite, PAH}, all zero except Δe on a rejected cell. */
+void orc_absorption_substep(orc_lya *L, int n_r, int n_z, const double *dt, double c_tilde,
+                            const double *rho_d, const double *kappa_F, int *redo, double *out)
+{
+    const int nn = L->n_nu, nc = n_r * n_z;
+    const double inv_ct = 1.0 / c_tilde;
+    const double e_floor = (FOUR_PI / C_LIGHT) * J_FLOOR_ABS;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nc; ++c) {
+        const double dts = dt[c];
+        const double ct_js = c_tilde * L->js[c];
+        const double ac = L->alpha_c[c];
+        const double pd_a = L->pd[c] * L->alpha[c];
+        const double aS = rho_d ? kappa_F[0] * rho_d[3 * c] : 0.0;
+        const double aC = rho_d ? kappa_F[1] * rho_d[3 * c + 1] : 0.0;
+        const double aP = rho_d ? kappa_F[2] * rho_d[3 * c + 2] : 0.0;
+        const double dt_lya = inv_ct / (ac + pd_a);
+        double *J = L->J + (size_t)c * nn, *Jo = L->J_old + (size_t)c * nn;
+        double *Hr = L->H_r + (size_t)c * nn, *HZ = L->H_z + (size_t)c * nn;
+        const double *Hro = L->H_r_old + (size_t)c * nn, *HZo = L->H_z_old + (size_t)c * nn;
+        const double *Hv = L->Hvoigt + (size_t)c * nn, *phi = L->phi + (size_t)c * nn;
+        const double *aF = L->alpha_F + (size_t)c * nn;
+        double sJ = 0.0, mr = 0.0, mz = 0.0, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < nn; ++k) {
+            const double dnu = L->dnu[k];
+            const double dJdt = ct_js * phi[k];
+            const double a_abs = ac + pd_a * Hv[k];
+            J[k] = jmax(absorption_update(Jo[k], dts, dJdt, a_abs, c_tilde), Jeps);
+            const double aFk = aF[k];
+            Hr[k] = absorption_update(Hro[k], dts, 0.0, aFk, c_tilde);
+            HZ[k] = absorption_update(HZo[k], dts, 0.0, aFk, c_tilde);
+            const double w = dnu * (FOUR_PI / C_LIGHT);
+            const double dr_k = w * (inv_ct * (Hro[k] - Hr[k]));
+            const double dz_k = w * (inv_ct * (HZo[k] - HZ[k]));
+            mr += dr_k;
+            mz += dz_k;
+            const double a_gas = jmax(aFk - aS - aC - aP, 0.0);
+            const double a_tot = a_gas + aS + aC + aP;
+            double fg = 1.0, fS = 0.0, fC = 0.0, fP = 0.0;
+            if (a_tot > 0.0) {
+                const double inv = 1 / a_tot;
+                fg = a_gas * inv; fS = aS * inv; fC = aC * inv; fP = aP * inv;
+            }
+            g[0] += fg * dr_k; g[1] += fg * dz_k;
+            g[2] += fS * dr_k; g[3] += fS * dz_k;
+            g[4] += fC * dr_k; g[5] += fC * dz_k;
+            g[6] += fP * dr_k; g[7] += fP * dz_k;
+            sJ += dnu * J[k];
+        }
+        const double e_new = (FOUR_PI / C_LIGHT) * sJ;
+        const double e_old = L->e_lya[c];
+        const double de = fabs(e_new - e_old) / (e_old + e_floor);
+        const int rej = (de > TOL_ABS && dts > TOL_ABS * dt_lya) || !isfinite(de);
+        redo[c] = rej;
+        double v[12] = {de, e_new, mr, mz, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7]};
+        if (rej) {
+            for (int m = 1; m < 12; ++m) v[m] = 0.0;
+            for (int k = 0; k < nn; ++k) { J[k] = Jo[k]; Hr[k] = Hro[k]; HZ[k] = HZo[k]; }
+        }
+        for (int m = 0; m < 12; ++m) out[(size_t)m * nc + c] = v[m];
+    }
+}
+
+
+/* ------------------------------------------------------------------------
+ * Lyα source terms of PrepareLyαSubCycling! (src/PhotoChemistryAndCooling/PhotoThermoChemistry.jl:2086-2155)
+ * ------------------------------------------------------------------------ */
+#define E_LYA_INV_4PI ((H_PLANCK * NU_LYA) * (1.0 / FOUR_PI))
+#define SIGMA_T      6.6524587321e-25
+#define SIGMA_RAMAN  2.1e-25
+#define SIGMA_OH     4.07e-18
+#define SIGMA_CH     0.05e-18
+#define A_2P2S       8.78e-7
+#define A_2S2P       1.597e-9
+#define A_2GAMMA     8.2206
+#define RHOD_IGNORE  (1.1 * (1e-13 * M_H))
+
+/* Storey & Hummer (1995) probability of Lyα emission per recombination, the tables of
+ * src/LymanAlphaEmission.jl:30-60 on log10 T (10 nodes) x log10 n_e (13 Case B / 9 Case A nodes). */
+static const double SH95_T[10] = {5e2, 1e3, 3e3, 5e3, 7.5e3, 1e4, 1.25e4, 1.5e4, 2e4, 3e4};
+static const double SH95_B[10][13] = {
+    {0.80793637, 0.90215197, 0.98157401, 0.9970534, 0.99933853, 0.99934845, 0.99967896, 0.99948547, 0.99931146, 0.99951894, 0.9997498, 0.99864818, 0.99372263},
+    {0.78439303, 0.87743841, 0.97423464, 0.99680114, 0.99960243, 0.99967816, 0.9997671, 0.99939133, 0.99922946, 0.99960973, 0.99937386, 0.99906812, 0.9954834},
+    {0.74159266, 0.82809599, 0.95734434, 0.99359891, 0.99892162, 0.99927343, 0.99979471, 0.99938909, 0.99899055, 0.99925979, 0.99957955, 0.99923525, 0.99703571},
+    {0.71946081, 0.80133228, 0.94581237, 0.99246182, 0.99848958, 0.99940329, 0.99949246, 0.99952048, 0.99981447, 0.99956726, 0.9993515, 0.99952024, 0.99692499},
+    {0.70117195, 0.7788453, 0.93441344, 0.99075058, 0.99831301, 0.99960223, 0.99940949, 0.99946884, 0.99948297, 0.99947146, 0.99965311, 0.99918254, 0.99690311},
+    {0.68808416, 0.76220739, 0.92503048, 0.9891268, 0.99828523, 0.9994074, 0.99944339, 0.99952601, 0.99932358, 0.99927528, 0.99938987, 0.99904232, 0.9949615},
+    {0.6775581, 0.74868826, 0.9176273, 0.98784252, 0.99835238, 0.99903921, 0.99977064, 0.99954296, 0.99930529, 0.99954743, 0.99944997, 0.99874439, 0.9916493},
+    {0.66862468, 0.73785038, 0.91033402, 0.98701405, 0.99817046, 0.99892625, 0.99953078, 0.99960556, 0.9993362, 0.99941558, 0.99944365, 0.99828039, 0.98817287},
+    {0.65534642, 0.72080463, 0.89823705, 0.98497224, 0.99798854, 0.9988133, 0.99929092, 0.99909416, 0.99933199, 0.9995549, 0.99924446, 0.99764512, 0.97960152},
+    {0.63566376, 0.69600394, 0.87952243, 0.98121463, 0.99716593, 0.99896144, 0.99905958, 0.9996252, 0.99953649, 0.99931002, 0.99872497, 0.99561256, 0.96391965}};
+static const double SH95_A[10][9] = {
+    {0.58042473, 0.59595911, 0.60335363, 0.58990186, 0.56623927, 0.53426043, 0.492528, 0.44534414, 0.40955195},
+    {0.54712958, 0.56465163, 0.57999443, 0.57241623, 0.5541246, 0.5276139, 0.49052298, 0.44606687, 0.4101595},
+    {0.48320245, 0.50151448, 0.52818381, 0.53008608, 0.52008701, 0.50298698, 0.47625175, 0.43924427, 0.40696841},
+    {0.44933274, 0.46714191, 0.49852558, 0.50475148, 0.49815515, 0.48505756, 0.46319941, 0.43069186, 0.40125666},
+    {0.42087234, 0.43810743, 0.47280049, 0.48268151, 0.47821493, 0.46794659, 0.4497736, 0.42097532, 0.39442497},
+    {0.40023971, 0.41675887, 0.45373431, 0.46591319, 0.46300415, 0.45454038, 0.43876097, 0.41249415, 0.38810578},
+    {0.3840314, 0.40007514, 0.43832241, 0.45257107, 0.45064954, 0.4434932, 0.42939783, 0.40511911, 0.38230403},
+    {0.37046215, 0.38617473, 0.42546609, 0.44143608, 0.44049428, 0.43394168, 0.42127472, 0.39854551, 0.37684759},
+    {0.34937867, 0.36397106, 0.40484801, 0.42349201, 0.42367822, 0.41860328, 0.40761256, 0.38699493, 0.36739625},
+    {0.31960139, 0.33302781, 0.37528203, 0.39805701, 0.39984039, 0.39602958, 0.38718651, 0.36963656, 0.3518518}};
+
+/* Bilinear interpolation on nodes xs[nx] x ys[ny] of f (row stride ny), at a point inside the grid. */
+static double bilinear(const double *xs, int nx, const double *ys, int ny, const double *f, double x, double y)
+{
+    int i = 0, j = 0;
+    while (i < nx - 2 && x > xs[i + 1]) ++i;
+    while (j < ny - 2 && y > ys[j + 1]) ++j;
+    const double t = (x - xs[i]) / (xs[i + 1] - xs[i]);
+    const double u = (y - ys[j]) / (ys[j + 1] - ys[j]);
+    const double f0 = f[i * ny + j] + u * (f[i * ny + j + 1] - f[i * ny + j]);
+    const double f1 = f[(i + 1) * ny + j] + u * (f[(i + 1) * ny + j + 1] - f[(i + 1) * ny + j]);
+    return f0 + t * (f1 - f0);
+}
+
+/* Prob_Lyα_SH95 (src/LymanAlphaEmission.jl:79-91): clamped to the table, flat beyond it. */
+double orc_prob_lya_sh95(double T, double ne, int caseB)
+{
+    double lt[10], ln[13];
+    for (int i = 0; i < 10; ++i) lt[i] = log10(SH95_T[i]);
+    const int nn = caseB ? 13 : 9;
+    for (int j = 0; j < nn; ++j) ln[j] = (double)(2 + j);   /* log10 n_e = 2, 3, ..., 14 (or 10) */
+    const double x = jmin(jmax(log10(T), lt[0]), lt[9]);
+    const double y = jmin(jmax(log10(ne), ln[0]), ln[nn - 1]);
+    return bilinear(lt, 10, ln, nn, caseB ? &SH95_B[0][0] : &SH95_A[0][0], x, y);
+}
+
+/* krecHII_A / krecHII_B (src/PhotoChemistryAndCooling/GasChemistryRates.jl:166-204). */
+double orc_krec_hii(double T, int caseB)
+{
+    T = jmin(jmax(T, 1.0), 1e9);
+    const double lam = 315614.0 / T;
+    if (caseB) return 2.753e-14 * pow(lam, 1.5) / pow(1 + pow(lam / 2.74, 0.407), 2.242);
+    return 1.269e-13 * pow(lam, 1.503) / pow(1 + pow(lam / 0.522, 0.47), 1.923);
+}
+
+/* ∫dν_js_recLyα (src/LymanAlphaEmission.jl:105-124): Case B with the on-the-spot approximation. */
+double orc_js_rec_lya(double T, double nHII, double ne, int ots)
+{
+    const double krec = orc_krec_hii(T, ots);
+    const double P = orc_prob_lya_sh95(T, ne, ots);
+    return P * krec * nHII * ne * E_LYA_INV_4PI;
+}
+
+/* ∫dν_js_collLyα (src/LymanAlphaEmission.jl:143-173). */
+double orc_js_coll_lya(double T, double nHI, double ne)
+{
+    const double T_ryd = (H_PLANCK * NU_LYC) / K_B;
+    const double T12 = (1 - 1.0 / 4.0) * T_ryd, T13 = (1 - 1.0 / 9.0) * T_ryd;
+    const double T14 = (1 - 1.0 / 16.0) * T_ryd, T15 = (1 - 1.0 / 25.0) * T_ryd;
+    T = jmax(T, 1.0);
+    const double inv_T = 1.0 / T;
+    const double T6 = jmin(T, 3e5) * 1e-6;
+    const double T6s = T6 * T6, T6c = T6s * T6;
+    const double Y12 = 0.267486 + 1.57257 * T6 - 6.44026 * T6s + 11.5401 * T6c;
+    const double Y13 = 0.0940821 + 2.90748 * T6 - 6.69528 * T6s + 7.86419 * T6c;
+    const double Y14 = 0.0415028 + 1.13673 * T6 - 3.97454 * T6s + 6.05941 * T6c;
+    const double Y15 = 0.032277 + 0.40856 * T6 - 1.34604 * T6s + 2.07558 * T6c;
+    const double q = (8.63e-6 / (2 * sqrt(T))) *
+                     (Y12 * exp(-T12 * inv_T) + Y13 * exp(-T13 * inv_T) + Y14 * exp(-T14 * inv_T) +
+                      Y15 * exp(-T15 * inv_T));
+    return q * nHI * ne * E_LYA_INV_4PI;
+}
+
+/* pd_Lyα (src/LymanAlphaDestruction.jl:43-97). */
+double orc_pd_lya(double nHII, double ne, double nHI, double nHeI, double nH2, double T, double z)
+{
+    T = jmin(jmax(T, 1.0), 1e5);
+    const double T100 = T / 100.0;
+    const double lT = log10(T), lT2 = lT * lT, lT3 = lT2 * lT, lT4 = lT3 * lT;
+    const double G2p2s = 3.1e-6 * (1 + z), G2s2p = 3 * G2p2s;
+    const double r25 = 2.5 / T;
+    const double C_e = 5e-5 * exp(-(r25 * r25)) * ne / (1 + pow(T100, 0.45));
+    const double C_HII = (6.98e-3 * (1 + 0.933 * lT) * exp(-pow(601 / T, 0.406)) / pow(T, 0.5) +
+                          pow(10.0, -7.69 + 1.21 * lT + 0.455 * lT2 - 0.199 * lT3 + 0.0182 * lT4)) * nHII;
+    const double C_HeI = 2.6e-9 * pow(T100, 0.315) * nHeI;
+    const double C_H2 = 1.7e-9 * pow(T100, 0.3) * nH2;
+    const double den = 8.9 * pow(T, -0.55) + 0.078 * pow(T, 0.35);
+    const double C_HI = (4.1e-11 * exp(-pow(1.6 / T, 3.0 / 4.0) - 0.15 * T) + (1.6e-10 + 1.57e-12 * pow(T, 1.1)) / (den * den)) * nHI;
+    const double C2s2p = C_e + C_HII + C_HeI + C_H2 + C_HI;
+    const double C2p2s = (1.0 / 3.0) * C2s2p;
+    const double R2p2s = (2.0 / 3.0) * A_2P2S + C2p2s + G2p2s;
+    const double R2s2p = A_2S2P + C2s2p + G2s2p;
+    const double pd_HI = R2p2s / ((A_LYA / A_2GAMMA) * (A_2GAMMA + R2s2p) + R2p2s);
+    double pd_H2 = 0.0;
+    if (nHI > 1e-20)
+        pd_H2 = jmin((33.0 * exp(-14540.0 / T) + 13.0 * exp(-15564.0 / T)) / (46.0 + 1e3 * exp(-pow(1e3 / T, 1.5))),
+                     1.3e-3) * (nH2 / nHI);
+    return jmin(pd_HI + pd_H2, 1.0);
+}
+
+/* The per-cell source terms of PrepareLyαSubCycling! for every cell: js, α_c (dust + gas
+ * absorption), α_c,F (dust + gas flux opacity, added to every bin of α_F), p_d.  chem is
+ * [8][n_cells] = T, n_HI, n_H2, n_HII, n_HeI, n_e, n_OHx, n_CHx; rho_d [n_cells][3];
+ * kappa = {κd_abs_Lyα, κd_F_Lyα} of silicate, graphite, PAH.  out [4][n_cells]. */
+void orc_lya_sources(int n_cells, const double *chem, const double *rho_d, const double *kappa, double z, int ots,
+                     int destruction, double *out)
+{
+    for (int c = 0; c < n_cells; ++c) {
+        const double T = chem[c], nHI = chem[n_cells + c], nH2 = chem[2 * n_cells + c];
+        const double nHII = chem[3 * n_cells + c], nHeI = chem[4 * n_cells + c], ne = chem[5 * n_cells + c];
+        const double nOH = chem[6 * n_cells + c], nCH = chem[7 * n_cells + c];
+        double rd[3];
+        for (int s = 0; s < 3; ++s) rd[s] = (rho_d[3 * c + s] > RHOD_IGNORE) ? rho_d[3 * c + s] : 0.0;
+        const double js = orc_js_rec_lya(T, nHII, ne, ots) + orc_js_coll_lya(T, nHI, ne);
+        const double ac_gas = nH2 * SIGMA_RAMAN + nOH * SIGMA_OH + nCH * SIGMA_CH;
+        const double acF_gas = ac_gas + ne * SIGMA_T;
+        const double ac = kappa[0] * rd[0] + kappa[1] * rd[1] + kappa[2] * rd[2] + ac_gas;
+        const double pd = destruction ? orc_pd_lya(nHII, ne, nHI, nHeI, nH2, T, z) : 0.0;
+        const double acF = kappa[3] * rd[0] + kappa[4] * rd[1] + kappa[5] * rd[2] + acF_gas;
+        out[c] = js;
+        out[n_cells + c] = ac;
+        out[2 * n_cells + c] = acF;
+        out[3 * n_cells + c] = pd;
+    }
+}

@@ -134,6 +134,19 @@ void orc_update_hvoigt(double *Hvoigt, int n_nu, int n_cells, const double *dnu_
 void orc_update_phi(double *phi, int n_nu, int n_cells, const double *dnu,
                     const double *Hvoigt, int kmin, int kmax);
 double orc_cfl_dt(const orc_geom *G, double c_tilde, const double *u_r, const double *u_z, double cfl);
+double orc_find_rsl(double c_old, double c_min, const orc_geom *G, const double *u_r, const double *u_z,
+                    const double *gamma_ad, const double *P, const double *rho, const double *rho_d,
+                    const double *xHII, const double *xH2, const double *T, const double *pd,
+                    const double *kappa_abs);
+void orc_absorption_substep(orc_lya *L, int n_r, int n_z, const double *dt, double c_tilde,
+                            const double *rho_d, const double *kappa_F, int *redo, double *out);
+double orc_prob_lya_sh95(double T, double ne, int caseB);
+double orc_krec_hii(double T, int caseB);
+double orc_js_rec_lya(double T, double nHII, double ne, int ots);
+double orc_js_coll_lya(double T, double nHI, double ne);
+double orc_pd_lya(double nHII, double ne, double nHI, double nHeI, double nH2, double T, double z);
+void orc_lya_sources(int n_cells, const double *chem, const double *rho_d, const double *kappa, double z, int ots,
+                     int destruction, double *out);
 void orc_hubble_flow(double r, double z, double umax, double rcl, double zc, double out[7]);
 double orc_cone_factor(double mu, double mucone, double taucone, double taucl,
                        int smooth, double dmu);

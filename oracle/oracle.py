@@ -93,6 +93,12 @@ def _declare(lib):
         "orc_update_phi": (None, [_dp, i, i, _dp, _dp, i, i]),
         "orc_cfl_dt": (d, [G, d, _dp, _dp, d]), "orc_hubble_flow": (None, [d, d, d, d, d, _dp]),
         "orc_cone_factor": (d, [d, d, d, d, i, d]),
+        "orc_find_rsl": (d, [d, d, G, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_absorption_substep": (None, [L, i, i, _dp, d, _dp, _dp, C.POINTER(i), _dp]),
+        "orc_prob_lya_sh95": (d, [d, d, i]), "orc_krec_hii": (d, [d, i]),
+        "orc_js_rec_lya": (d, [d, d, d, i]), "orc_js_coll_lya": (d, [d, d, d]),
+        "orc_pd_lya": (d, [d, d, d, d, d, d, d]),
+        "orc_lya_sources": (None, [i, _dp, _dp, _dp, d, i, i, _dp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -243,6 +249,18 @@ class OracleState:
         load().orc_edge_spectra(C.byref(self.lya), self.n_r, self.n_z, ptr(raw), ptr(norm))
         return raw, norm
 
+    def absorption_substep(self, dt, c_tilde, rho_d=None, kappa_F=(0.0, 0.0, 0.0)):
+        """The Lyα block of the RHD absorption sub-step for every cell; dt scalar or (N_Z, N_r).
+        Returns (redo (N_Z, N_r) ints, out (12, N_Z, N_r))."""
+        dtc = f64(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.n_z, self.n_r)))
+        redo = np.zeros((self.n_z, self.n_r), dtype=np.int32)
+        out = np.zeros((12, self.n_z, self.n_r))
+        kF = f64(kappa_F)
+        load().orc_absorption_substep(C.byref(self.lya), self.n_r, self.n_z, ptr(dtc), c_tilde,
+                                      ptr(f64(rho_d)) if rho_d is not None else None, ptr(kF),
+                                      redo.ctypes.data_as(C.POINTER(C.c_int)), ptr(out))
+        return redo, out
+
     def step(self, c_tilde=None, dt=None, z_center=None, bc_z_bot=None):
         out = np.zeros(8)
         load().orc_static_cloud_step(self.c_tilde if c_tilde is None else c_tilde, self.dt if dt is None else dt,
@@ -250,3 +268,27 @@ class OracleState:
                                      self.Zc if z_center is None else z_center,
                                      self.bc_z_bot if bc_z_bot is None else bc_z_bot, ptr(out))
         return out
+
+
+def find_rsl(c_old, c_min, geom, u_r, u_z, gamma_ad, P, rho, rho_d, xHII, xH2, T, pd, kappa_abs):
+    """FindRSL_LyαRT on the oracle (cell arrays (N_Z, N_r), rho_d (N_Z, N_r, 3))."""
+    g = OracleGeom(geom)
+    arrs = [f64(a) for a in (u_r, u_z, gamma_ad, P, rho, rho_d, xHII, xH2, T, pd, kappa_abs)]
+    return load().orc_find_rsl(c_old, c_min, C.byref(g.s), *[ptr(a) for a in arrs])
+
+
+def cfl_dt(geom, c_tilde, u_r, u_z, cfl):
+    """GetRHDCFLTimeStep on the oracle."""
+    g = OracleGeom(geom)
+    return load().orc_cfl_dt(C.byref(g.s), c_tilde, ptr(f64(u_r)), ptr(f64(u_z)), cfl)
+
+
+def lya_sources(chem, rho_d, kappa, z=0.0, ots=True, destruction=True):
+    """The per-cell source terms of PrepareLyαSubCycling! on the oracle: chem (8, N_Z, N_r) =
+    T, n_HI, n_H2, n_HII, n_HeI, n_e, n_OHx, n_CHx; returns (4, N_Z, N_r) = js, α_c, α_c,F, p_d."""
+    chem = f64(chem)
+    n = chem[0].size
+    out = np.zeros((4,) + chem.shape[1:])
+    rd = f64(rho_d) if rho_d is not None else np.zeros(chem.shape[1:] + (3,))
+    load().orc_lya_sources(n, ptr(chem), ptr(rd), ptr(f64(kappa)), z, int(ots), int(destruction), ptr(out))
+    return out

@@ -177,7 +177,9 @@ def test_reverse_interfaces_match_dual_passes(lut, case):
             w = getattr(rev, group)[k]
             scale = np.max(np.abs(v))
             err = np.max(np.abs(w - v)) / scale if scale > 0 else np.max(np.abs(w))
-            assert err <= 1e-9, (case, group, k, err)
+            print(f"[reverse vs dual] {case} {group} {k}: {err:.1e}")
+            # two roundings of the same derivative: a transcription slip shows as O(1)
+            assert err <= 1e-8, (case, group, k, err)
 
 
 def test_checkpoint_interval_and_final_state(lut):
@@ -201,7 +203,7 @@ def test_checkpoint_interval_and_final_state(lut):
                 assert bitdiff(getattr(res, group)[k], v) == 0, (every, group, k)
 
 
-@pytest.fixture(scope="module", params=[200.0, 3200.0])
+@pytest.fixture(scope="module", params=[10.0, 200.0, 800.0, 3200.0])
 def expanding(request, lut):
     u_max = request.param
     return developed_state(W.expanding_cloud(128, 128, 128, tau0=1e7, u_max=u_max, sigma_ln=0.5, lut=lut))
@@ -219,16 +221,20 @@ def _fd_pair(f, x0, steps):
     return [(f(x0 + h) - f(x0 - h)) / (2 * h) for h in steps]
 
 
-def _per_cell_check(label, cells, grad, fd_of_cell, x0, rel_steps, cloud):
+def _per_cell_check(label, cells, grad, fd_of_cell, x0, rel_steps, cloud, obj=0.0):
     """Adjoint vs central differences at two steps.  Where the two differences disagree by more
     than the tolerance a kink (limiter switch, a clamp of the exponential-fitting exponent) lies
     within the larger step: such cells are reported and left out.  On the others the adjoint
-    must agree with the smaller-step difference to FD_TOL, relative to max(|grad|, 1e-3 max|grad|)."""
+    must agree with the smaller-step difference to FD_TOL, relative to max(|grad|, 1e-3 max|grad|,
+    the resolution of the difference): a central difference with step h of an objective of size
+    |obj| cannot resolve derivatives below about 8 ε |obj| / h."""
     scale = np.max(np.abs(grad))
+    eps = np.finfo(float).eps
     worst, used = 0.0, 0
     for (j, i) in cells:
         ad = grad[j, i]
-        floor = max(abs(ad), 1e-3 * scale)
+        resolution = 8 * eps * abs(obj) / (rel_steps[1] * abs(x0[j, i])) / FD_TOL
+        floor = max(abs(ad), 1e-3 * scale, resolution)
         fds = fd_of_cell(j, i, [r * x0[j, i] for r in rel_steps])
         smooth = abs(fds[0] - fds[1]) <= FD_TOL * floor
         e = abs(fds[1] - ad) / floor
@@ -264,7 +270,8 @@ def test_expanding_cloud_density_and_temperature_gradients(expanding, lut):
             return _objective(cloud, n_mf, mf, None, A.cloud_with(cloud, n_H=nH, lut=lut))
         return _fd_pair(f, nH0[j, i], steps)
 
-    worst_nH, used_nH = _per_cell_check("dMF/dnH", cells, g_mf["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud)
+    worst_nH, used_nH = _per_cell_check("dMF/dnH", cells, g_mf["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud,
+                                        obj=res_mf.objective)
 
     n_sp = 64
     k = np.arange(nn)
@@ -280,7 +287,8 @@ def test_expanding_cloud_density_and_temperature_gradients(expanding, lut):
             return _objective(cloud, n_sp, None, sp, A.cloud_with(cloud, T=T, lut=lut))
         return _fd_pair(f, T0[j, i], steps)
 
-    worst_T, used_T = _per_cell_check("dS/dT", cells, g_sp["T"], fd_T, T0, (1e-4, 1e-5), cloud)
+    worst_T, used_T = _per_cell_check("dS/dT", cells, g_sp["T"], fd_T, T0, (1e-4, 1e-5), cloud,
+                                      obj=res_sp.objective)
 
     hL = 1e-5
     fd = (_objective(cloud, n_mf, mf, None, A.cloud_with(cloud, L_scale=1 + hL, lut=lut))
@@ -316,7 +324,7 @@ def test_rhd_like_state_radial_force_gradient(lut):
             return _objective(cloud, n, mf, None, A.cloud_with(cloud, n_H=nH, lut=lut))
         return _fd_pair(f, nH0[j, i], steps)
 
-    worst, used = _per_cell_check("dF/dnH", cells, g["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud)
+    worst, used = _per_cell_check("dF/dnH", cells, g["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud, obj=res.objective)
     hL = 1e-5
     fd = (_objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 + hL, lut=lut))
           - _objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 - hL, lut=lut))) / (2 * hL)
@@ -324,3 +332,39 @@ def test_rhd_like_state_radial_force_gradient(lut):
     eL = abs(fd - ad) / max(abs(fd), 1e-300)
     print(f"[adjoint fd] rhd dF/dlnL fd {fd:+.8e} adj {ad:+.8e} rel {eL:.1e}; n_H worst {worst:.1e} over {used}")
     assert used >= 9 and worst <= FD_TOL and eL <= FD_TOL, (worst, used, eL)
+
+
+def test_rhd_like_state_full_size_radial_force_gradient(lut):
+    """The integrated radial force Σ_s M_F_radial_net(s) over 4 substeps of the RHD-like state at
+    its full size (256x512 r-Z, 128 bins), checkpointed every 2 substeps: d/dn_H on 4 random cells
+    and d/d ln L against central differences."""
+    cloud = developed_state(W.rhd_like_cloud(256, 512, 128, lut=lut))
+    n = 4
+    mf = np.zeros((n, 3))
+    mf[:, 1] = 1.0
+    res, ctx = _adjoint(cloud, n, mf, None, checkpoint_every=2)
+    ctx.close()
+    g = A.physical_gradients(cloud, res, lut, with_T=False)
+    obj = res.objective
+    del res
+    nH0 = A.density(cloud)
+    rng = np.random.Generator(np.random.PCG64(29))
+    cells = np.argwhere(np.ones_like(nH0, dtype=bool)[2:-2, 2:-2]) + 2
+    cells = cells[rng.choice(len(cells), size=4, replace=False)]
+
+    def fd_nH(j, i, steps):
+        def f(x):
+            nH = nH0.copy()
+            nH[j, i] = x
+            return _objective(cloud, n, mf, None, A.cloud_with(cloud, n_H=nH, lut=lut))
+        return _fd_pair(f, nH0[j, i], steps)
+
+    worst, used = _per_cell_check("dF/dnH", cells, g["n_H"], fd_nH, nH0, (1e-4, 1e-5), cloud, obj=obj)
+    hL = 1e-5
+    fd = (_objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 + hL, lut=lut))
+          - _objective(cloud, n, mf, None, A.cloud_with(cloud, L_scale=1 - hL, lut=lut))) / (2 * hL)
+    ad = g["L"] * cloud.params["L_lya"]
+    eL = abs(fd - ad) / max(abs(fd), 1e-300)
+    print(f"[adjoint fd] rhd 256x512x128 dF/dlnL fd {fd:+.8e} adj {ad:+.8e} rel {eL:.1e}; "
+          f"n_H worst {worst:.1e} over {used}")
+    assert used >= 3 and worst <= FD_TOL and eL <= FD_TOL, (worst, used, eL)

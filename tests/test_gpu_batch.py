@@ -16,6 +16,7 @@ from lydion_amd import m1lya as M
 from lydion_amd.batch import M1LyaBatch
 from lydion_amd.static_cloud import developed_state
 from oracle.oracle import OracleState
+from tests import physics_cases as P
 from tests.helpers import bitdiff, small_cloud
 
 pytestmark = pytest.mark.gpu
@@ -94,14 +95,27 @@ def test_outflow_bottom_and_reflecting_right(lut):
     _check_history(hb, h64, "outflow-bottom")
 
 
+def _spectrum_err(cloud, b, ctx, idx=0):
+    """Max relative error of the escaping spectrum (the per-bin terms of f_esc) over the bins
+    above 1e-3 of its peak."""
+    _, _, s32 = P.escaping_spectrum(cloud, b.download(idx, "H_r"), b.download(idx, "H_z"))
+    _, _, s64 = P.escaping_spectrum(cloud, ctx.download("H_r"), ctx.download("H_z"))
+    big = s64 > 1e-3 * s64.max()
+    return float(np.max(np.abs(s32[big] - s64[big]) / s64[big]))
+
+
 def test_cone_dust_cloud_256x256x128_vs_fp64(lut):
-    """The full-size cloud: 8 steps from the developed state."""
+    """The full-size cloud: 8 steps from the developed state; M_F, f_esc, J energy and the
+    escaping spectrum."""
     cloud = developed_state(L.cone_dust_cloud_256x256x128(lut=lut))
     h64, ctx = _fp64_history(cloud, 8)
     b = M1LyaBatch([cloud])
     hb = b.run_steps(8)[:, 0, :]
     _check_history(hb, h64, "cone_dust_256x256x128")
     assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
+    es = _spectrum_err(cloud, b, ctx)
+    print(f"cone 256x256x128: escaping spectrum max rel err {es:.2e}")
+    assert es <= FP32_TOL
 
 
 def _mixed_clouds(lut):
@@ -257,6 +271,25 @@ def test_rhd_like_state_vs_fp64(lut):
     h64, ctx = _fp64_history(cloud, 20)
     _check_history(hb[:, 0, :], h64, "rhd-like")
     assert _energy_err(cloud, b.download(0, "J"), ctx.download("J")) <= FP32_TOL
+    ctx.close()
+    b.close()
+
+
+def test_rhd_like_state_full_size_vs_fp64(lut):
+    """The RHD-like state at its full size (256x512 r-Z, 128 bins, FP32 state of 200 MB): 24
+    substeps from a developed field, every step's M_F and f_esc, the J energy and the escaping
+    spectrum against the FP64 step."""
+    from lydion_amd import workloads as W
+    cloud = developed_state(W.rhd_like_cloud(256, 512, 128, lut=lut))
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(24)
+    h64, ctx = _fp64_history(cloud, 24)
+    _check_history(hb[:, 0, :], h64, "rhd-like 256x512x128")
+    ee = _energy_err(cloud, b.download(0, "J"), ctx.download("J"))
+    es = _spectrum_err(cloud, b, ctx)
+    print(f"rhd 256x512x128, 24 substeps: M_F rel {_relerr(hb[-1, 0, 0], h64[-1, 0]):.2e}, "
+          f"f_esc rel {_relerr(hb[-1, 0, 4], h64[-1, 4]):.2e}, J energy {ee:.2e}, spectrum {es:.2e}")
+    assert ee <= FP32_TOL and es <= FP32_TOL
     ctx.close()
     b.close()
 

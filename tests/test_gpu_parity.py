@@ -154,6 +154,23 @@ def test_fused_step_bit_exact(lut, case):
     assert bitdiff(ctx.download("J_old"), o["J_old"]) == 0
 
 
+def test_fused_step_bit_exact_cone_cloud_256x256x128(lut):
+    """The full-size cone cloud (τ0 = 1e8, τ_abs = 1, 128 bins) from a developed field: two fused
+    steps bit-identical to the oracle, so the grid-stride column kernel and the advection blocks
+    are compared at the benchmark's size."""
+    from lydion_amd.static_cloud import developed_state
+    cloud = developed_state(L.cone_dust_cloud_256x256x128(lut=lut))
+    ctx, o = _pair(cloud)
+    for s in range(2):
+        so = o.step()
+        sg = M.step(ctx, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot)
+        assert (sg["kmin"], sg["kmax"]) == (so[5], so[6]), f"step {s}"
+        for name, idx in (("M_F", 0), ("M_F_radial_net", 1), ("M_F_radial_out", 2), ("L_Lya", 3), ("f_esc", 4)):
+            assert abs(sg[name] - so[idx]) <= RED_TOL * max(abs(so[idx]), 1e-300), (s, name, sg[name], so[idx])
+    _assert_state_equal(ctx, o)
+    ctx.close()
+
+
 def test_run_steps_matches_single_steps(lut):
     cloud = L.static_cloud_32x32x64(lut=lut)
     c1 = M.M1LyaContext(cloud.lya, cloud.geometry)
