@@ -425,6 +425,21 @@ int  m1lya_batch_reset_kernel_stats(m1lya_batch *b);
  * No counterpart in the reference, which has no adjoint. */
 int m1lya_set_adjoint_mode(m1lya_ctx *ctx, int mode);
 
+/* ---- allocation-free entry point ------------------------------------------
+ * No counterpart in the reference: FindActiveBins (src/UpdateLyaJandH.jl:47-97, as
+ * m1lya_find_active_bins) for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  J [N_ν, N_r, N_Z],
+ * dnu [N_ν] and dV [N_r, N_Z] are device pointers the caller owns; kmin/kmax are host
+ * outputs (0-based).  The partial sums use a fixed device array of the library (4 MiB per
+ * GPU, kept until the process ends), so N_ν is limited to 256 (M1LYA_ERR_DIMS beyond).
+ * Runs on the null stream and synchronises it; no device memory is allocated.  Calls that
+ * overlap on one GPU share that array and race.  The result is bit-identical to
+ * m1lya_find_active_bins on the same state.  m1lya_noalloc_last_error returns the message
+ * of its last failure on the calling thread. */
+int m1lya_find_active_bins_noalloc(const double *J, const double *dnu, const double *dV, int n_nu, int n_r,
+                                   int n_z, int kcore_min, int kcore_max, int *kmin, int *kmax);
+int m1lya_noalloc_last_error(char *buf, size_t len);
+
 /* ---- instrumentation ------------------------------------------------- */
 /* When on, every kernel launch is bracketed by HIP events on the context's
  * stream; m1lya_kernel_stats() reports per-kernel launch counts and total

@@ -82,6 +82,7 @@ EXPORTED = [
     "m1lya_batch_autotune", "m1lya_batch_set_graph", "m1lya_batch_synchronize", "m1lya_batch_set_profiling",
     "m1lya_batch_kernel_stats_count", "m1lya_batch_kernel_stats", "m1lya_batch_reset_kernel_stats",
     "m1lya_batch_set_voigt_lut", "m1lya_batch_prepare_subcycling",
+    "m1lya_find_active_bins_noalloc", "m1lya_noalloc_last_error",
     "m1lya_cfl_time_step", "m1lya_find_rsl", "m1lya_absorption_substep", "m1lya_prepare_subcycling_chem",
 ]
 
@@ -127,6 +128,8 @@ def load() -> C.CDLL:
         "m1lya_step": (i, [vp, d, d, d, i, i, i, _dp]),
         "m1lya_run_steps": (i, [vp, i, d, d, d, i, i, i, _dp]),
         "m1lya_run_until_steady": (i, [vp, i, d, d, d, d, i, i, i, _dp, C.POINTER(C.c_int)]),
+        "m1lya_find_active_bins_noalloc": (i, [vp, vp, vp, i, i, i, i, i, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "m1lya_noalloc_last_error": (i, [C.c_char_p, sz]),
         "m1lya_cfl_time_step": (i, [vp, d, _dp, _dp, d, _dp]),
         "m1lya_find_rsl": (i, [vp, d, d, C.POINTER(RslInputs), _dp]),
         "m1lya_absorption_substep": (i, [vp, _dp, d, d, _dp, _dp, C.POINTER(C.c_int), _dp]),

@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libm1lya.so")
-SOURCES = ["m1lya_kernels.hip", "m1lya_api.hip", "m1lya_batch.hip", "m1lya_adjoint.hip"]
+SOURCES = ["m1lya_kernels.hip", "m1lya_api.hip", "m1lya_batch.hip", "m1lya_adjoint.hip", "m1lya_noalloc.hip"]
 ARCH = os.environ.get("M1LYA_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [
