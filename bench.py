@@ -69,7 +69,7 @@ def parse():
     p.add_argument("--n-r", type=int, default=256)
     p.add_argument("--n-z", type=int, default=256)
     p.add_argument("--n-nu", type=int, default=128)
-    p.add_argument("--accuracy-steps", type=int, default=8)
+    p.add_argument("--accuracy-steps", type=int, default=50)
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU-oracle sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -89,26 +89,36 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(cloud_factory, seconds: float):
-    """FP64 oracle (C restatement of the reference, OpenMP over cells) on the host cores."""
-    from oracle.oracle import OracleState
+def cpu_baseline(cloud_factory, seconds: float, single_seconds: float = 5.0):
+    """FP64 oracle (C restatement of the reference, gcc -O3 -march=x86-64-v3, OpenMP over cells)
+    on the host cores, and on one core for a shorter sample."""
+    from oracle.oracle import OracleState, load
 
     from lydion_amd.static_cloud import developed_state
     cloud = developed_state(cloud_factory())
     o = OracleState(cloud)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o.step()  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.step()
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
     upd = cloud.lya.n_nu * cloud.lya.n_r * cloud.lya.n_z
+
+    def sample(limit):
+        o.step()  # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            o.step()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= limit:
+                return n, el
+
+    n, el = sample(seconds)
+    load().orc_set_threads(1)
+    n1, el1 = sample(single_seconds)
+    load().orc_set_threads(threads)
     return {"value": upd * n / el, "unit": "cell-freq updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} FP64 steps of the same {cloud.lya.n_r}x{cloud.lya.n_z}x{cloud.lya.n_nu} workload "
-                      f"({el:.1f} s, OMP_NUM_THREADS={threads})"}
+                      f"({el:.1f} s, OMP_NUM_THREADS={threads})",
+            "single_thread": {"value": upd * n1 / el1, "cores": 1,
+                              "sample": f"{n1} FP64 steps on one thread ({el1:.1f} s)"}}
 
 
 def copy_calibration():
@@ -181,7 +191,9 @@ def run_fp32(clouds, steps, warmup, autotune):
 
 
 def accuracy(cloud, nsteps):
-    """FP32 batched vs FP64 plain from the same state after nsteps steps."""
+    """FP32 batched vs FP64 plain from the same state after nsteps steps: M_F and f_esc at every
+    step, the frequency-integrated J, and the emergent (escaping) spectrum of the last step."""
+    from lydion_amd import adjoint as A
     from lydion_amd import m1lya as M
     from lydion_amd.batch import M1LyaBatch
     ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
@@ -191,23 +203,51 @@ def accuracy(cloud, nsteps):
     rel = lambda a, r: float(np.max(np.abs(a - r) / np.maximum(np.abs(r), 1e-300)))
     w = cloud.lya["dnu"][None, None, :] * cloud.geometry.dV[:, :, None]
     J64, J32 = ctx.download("J"), b.download(0, "J")
+    L_sim = float(h64[-1, 3]) * (0.5 if cloud.bc_z_bot == 0 else 1.0)
+    S64 = A.escaping_spectrum(cloud.geometry, ctx.download("H_r"), ctx.download("H_z"), L_sim, cloud.bc_z_bot)
+    S32 = A.escaping_spectrum(cloud.geometry, b.download(0, "H_r"), b.download(0, "H_z"), L_sim, cloud.bc_z_bot)
+    big = S64 > 1e-3 * S64.max()
     out = {"steps": nsteps, "M_F_max_rel_err": rel(h32[:, 0], h64[:, 0]),
            "f_esc_max_rel_err": rel(h32[:, 4], h64[:, 4]),
            "J_energy_rel_err": float(np.sum(np.abs(J32 - J64) * w) / np.sum(np.abs(J64) * w)),
+           "spectrum_max_rel_err": rel(S32[big], S64[big]),
+           "spectrum_bins_compared": int(big.sum()),
            "M_F_fp64_last": float(h64[-1, 0]), "f_esc_fp64_last": float(h64[-1, 4])}
     ctx.close()
     b.close()
     return out
 
 
+CUS, CLOCK_HZ = 256, 2.4e9   # MI355X: compute units, peak engine clock
+
+
+def load_valu(kernel: str, workload: str):
+    """VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, profiles/)."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{workload}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["SQ_INSTS_VALU"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def roofline(stats, dom, upd_per_step, steps, workload):
+    """The dominant kernel against the HBM roofline.  Beside it, the share of its duration that
+    issuing its VALU instructions takes (one wave64 VALU instruction per CU per cycle): where
+    that share exceeds the HBM fraction, instruction issue, not bandwidth, limits the kernel."""
     launches, total_ms = stats[dom]
     avg_s = total_ms * 1e-3 / launches
     achieved = ALGO_BYTES[dom] * upd_per_step / avg_s / 1e9
+    valu = load_valu(dom, workload)
+    issue = None if valu is None else valu / (CUS * CLOCK_HZ) / avg_s
+    frac = achieved / HBM_PEAK_GBS
     return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(dom, workload),
+            "frac": round(frac, 4), "traffic": load_traffic(dom, workload),
             "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * upd_per_step,
             "avg_launch_ms": round(1e3 * avg_s, 4),
+            "valu_issue_frac": None if issue is None else round(issue, 4),
+            "valu_insts_per_update": None if valu is None else round(valu / upd_per_step, 3),
+            "limiter": None if issue is None else ("valu-issue" if issue > frac else "hbm"),
             "kernels_ms_per_step": {k: round(v[1] / steps, 4) for k, v in sorted(stats.items())}}
 
 

@@ -16,13 +16,13 @@ Usage in src/StaticCloudTest.jl (after GenerateLyαArrays / UpdateFreqFactors!):
 """
 module M1LyaGPU
 
-export attach!, download_state!, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
+export attach!, download_state!, upload_state!, edge_spectra, AdvectLyα!, LyαEmissionAbsorptionUpdate!,
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
        Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!,
        set_voigt_lut!, prepare_subcycling!, set_adjoint_mode!, run_until_steady!, GetRHDCFLTimeStep,
-       FindRSL_LyαRT, absorption_substep!
+       FindRSL_LyαRT, absorption_substep!, run_MF
 
-const libm1lya = joinpath(@__DIR__, "..", "libm1lya.so")
+const libm1lya = joinpath(@__DIR__, "..", "..", "libm1lya.so")
 
 # enum m1lya_field (include/m1lya.h)
 const F_J, F_J_OLD, F_H_R, F_H_Z, F_H_R_OLD, F_H_Z_OLD = 0, 1, 2, 3, 4, 5
@@ -116,6 +116,33 @@ function attach!(Lyα, g, T, u_r_star, u_Z_star, ∂u_r_∂r, ∂u_r_∂Z, ∂u_
     end
     CONTEXTS[Lyα] = ctx
     return ctx
+end
+
+"""Pushes the LyαArrays' J and H to the device, with the *_old arrays equal to them, and
+recomputes e_Lyα there: the restart of src/RHDSolver_SingleStar.jl:1359-1372 after the reference
+has read /Checkpoint's J_Lya, H_r_Lya, H_Z_Lya into LyαState."""
+function upload_state!(Lyα)
+    ctx = ctxof(Lyα)
+    for (f, A) in ((F_J, Lyα.J_Lyα), (F_J_OLD, Lyα.J_Lyα), (F_H_R, Lyα.H_r_Lyα), (F_H_Z, Lyα.H_Z_Lyα),
+                   (F_H_R_OLD, Lyα.H_r_Lyα), (F_H_Z_OLD, Lyα.H_Z_Lyα))
+        upload!(ctx, f, A)
+    end
+    check(ccall((:m1lya_update_e_lya, libm1lya), Cint, (Ptr{Cvoid},), ctx), ctx)
+    download!(ctx, F_E_LYA, Lyα.e_Lyα)
+    return Lyα
+end
+
+"""The edge spectra SaveTimeStepHDF5 stores (src/SaveHDF5Data_StaticCloudTests.jl:392-420), from the
+device state: (raw, normalised), each [N_edge, N_ν]."""
+function edge_spectra(Lyα)
+    ctx = ctxof(Lyα)
+    ne = ccall((:m1lya_edge_count, libm1lya), Cint, (Ptr{Cvoid},), ctx)
+    N_ν = size(Lyα.J_Lyα, 1)
+    raw = zeros(N_ν, ne)
+    norm = zeros(N_ν, ne)
+    check(ccall((:m1lya_edge_spectra, libm1lya), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Csize_t),
+                ctx, raw, norm, length(raw)), ctx)
+    return permutedims(raw), permutedims(norm)
 end
 
 """Copies the device state back into the LyαArrays (SaveTimeStepHDF5, /Checkpoint)."""
@@ -278,6 +305,16 @@ function run_batch!(b::Ptr{Cvoid}, nsteps::Integer)
     hist = Array{Float64}(undef, 8, n, nsteps)
     bcheck(ccall((:m1lya_batch_run_steps, libm1lya), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), b, nsteps, hist), b)
     return hist
+end
+
+# --- the M_F of each of n fused steps: the primal whose rrule (ext/M1LyaGPUChainRulesCoreExt.jl)
+#     replays the steps through m1lya_adjoint_run ---
+function run_MF(Lyα, nsteps::Integer, c̃::Float64, Δt::Float64, Z_center::Float64; BC_Z_bot::Symbol=:Reflect)
+    ctx = ctxof(Lyα)
+    hist = Array{Float64}(undef, 8, nsteps)
+    check(ccall((:m1lya_run_steps, libm1lya), Cint, (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Cint, Cint,
+                Cint, Ptr{Float64}), ctx, nsteps, c̃, Δt, Z_center, bc(:Outflow), bc(BC_Z_bot), bc(:Outflow), hist), ctx)
+    return hist[1, :]
 end
 
 # --- reverse-mode gradient of n fused steps (m1lya_adjoint_run) ---

@@ -1570,3 +1570,17 @@ void orc_lya_sources(int n_cells, const double *chem, const double *rho_d, const
         out[3 * n_cells + c] = pd;
     }
 }
+
+
+/* Number of OpenMP threads the oracle's loops use (the CPU baseline's single-thread sample). */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}

@@ -140,6 +140,7 @@ double orc_find_rsl(double c_old, double c_min, const orc_geom *G, const double 
                     const double *kappa_abs);
 void orc_absorption_substep(orc_lya *L, int n_r, int n_z, const double *dt, double c_tilde,
                             const double *rho_d, const double *kappa_F, int *redo, double *out);
+void orc_set_threads(int n);
 double orc_prob_lya_sh95(double T, double ne, int caseB);
 double orc_krec_hii(double T, int caseB);
 double orc_js_rec_lya(double T, double nHII, double ne, int ots);

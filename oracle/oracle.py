@@ -99,6 +99,7 @@ def _declare(lib):
         "orc_js_rec_lya": (d, [d, d, d, i]), "orc_js_coll_lya": (d, [d, d, d]),
         "orc_pd_lya": (d, [d, d, d, d, d, d, d]),
         "orc_lya_sources": (None, [i, _dp, _dp, _dp, d, i, i, _dp]),
+        "orc_set_threads": (None, [i]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
