@@ -40,7 +40,8 @@ typedef struct m1lya_ctx m1lya_ctx;
 enum m1lya_status {
     M1LYA_OK = 0,
     M1LYA_ERR_ARG = -1,        /* bad argument (NULL pointer, wrong count, bad enum) */
-    M1LYA_ERR_DIMS = -2,       /* unsupported dimensions (N_r < 3, N_Z < 3, N_ν < 3) */
+    M1LYA_ERR_DIMS = -2,       /* unsupported dimensions (N_r < 3, N_Z < 3, N_ν < 3;
+                                  N_ν > 576 for a context, > 256 for a batch)         */
     M1LYA_ERR_HIP = -3,        /* HIP runtime error (allocation, launch)            */
     M1LYA_ERR_NONFINITE = -4,  /* a fused step wrote a non-finite J, H_r or H_Z, or its
                                   tridiagonal system was singular                      */
@@ -332,14 +333,15 @@ int m1lya_prepare_subcycling_chem(m1lya_ctx *ctx, const double *chem, const doub
  * (limiters, floors, clamps) the derivative of the branch taken is returned.
  * Memory: states are checkpointed every checkpoint_every steps (0: ceil(√n))
  * and one segment is replayed at a time: about (n/S + 2 S + 12) state-sized
- * FP64 arrays of N_ν N_r N_Z each (S = checkpoint interval). */
+ * FP64 arrays of N_ν N_r N_Z each (S = checkpoint interval).  N_ν <= 256
+ * (M1LYA_ERR_DIMS beyond). */
 int m1lya_adjoint_run(m1lya_ctx *ctx, int n_steps, double c_tilde, double dt, double z_center,
                       int bc_r_right, int bc_z_bot, int bc_z_top, const double *mf_seed,
                       const double *spec_seed, int checkpoint_every, double *objective, double *history,
                       double *grad_state, double *grad_cell, double *grad_bin);
 
 /* ---- mixed-precision batched step ---------------------------------------
- * A batch advances many clouds of uneven size (N_r, N_Z >= 3, 3 <= N_ν <= 128),
+ * A batch advances many clouds of uneven size (N_r, N_Z >= 3, 3 <= N_ν <= 256),
  * optical depth and frequency resolution by the same fused step as m1lya_step
  * (the RT-only loop body of src/StaticCloudTest.jl:357-392), one launch per step
  * for the whole batch.  Precision: J, H_r, H_Z, α_F, Hvoigt and ϕ_ν are stored

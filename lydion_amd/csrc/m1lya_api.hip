@@ -478,8 +478,8 @@ int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
         g_last_error = "need N_ν >= 3, N_r >= 3, N_Z >= 3 (src/SpatialGrid.jl:629-630)";
         return M1LYA_ERR_DIMS;
     }
-    if (n_nu > 256) {
-        g_last_error = "N_ν > 256 is not supported by the column kernel";
+    if (n_nu > kMaxBinsFP64) {
+        g_last_error = "N_ν > " + std::to_string(kMaxBinsFP64) + " is not supported by the column kernel";
         return M1LYA_ERR_DIMS;
     }
     if (precision != M1LYA_FP64) {
@@ -510,7 +510,7 @@ int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
         c->ivr = dalloc(c, (size_t)(n_r + 1) * 5); c->ivz = dalloc(c, (size_t)(n_z + 1) * 5);
         c->nu = dalloc(c, n_nu + 1); c->nuC = dalloc(c, n_nu); c->dnu = dalloc(c, n_nu);
         c->inv_dnu = dalloc(c, n_nu); c->inv_nuC = dalloc(c, n_nu);
-        const int W = column_width();
+        const int W = column_width(n_nu);
         const int ngroups = (int)((nc + W - 1) / W);
         c->col_blocks = column_blocks(n_nu, ngroups);
         c->act_blocks = (int)std::min<size_t>(nc, 1024);
@@ -1251,6 +1251,9 @@ int m1lya_adjoint_run(m1lya_ctx* c, int n_steps, double c_tilde, double dt, doub
         if (n_steps < 1) fail(c, M1LYA_ERR_ARG, "n_steps < 1");
         if (checkpoint_every < 0) fail(c, M1LYA_ERR_ARG, "checkpoint_every < 0");
         if (!(c_tilde > 0.0) || !(dt > 0.0)) fail(c, M1LYA_ERR_ARG, "c_tilde and dt must be > 0");
+        // the reverse column kernel keeps 16 rows of N_ν doubles in LDS (32 KiB at 256 bins)
+        if (c->nn > kMaxBinsAdjoint)
+            fail(c, M1LYA_ERR_DIMS, "the adjoint supports N_ν <= " + std::to_string(kMaxBinsAdjoint));
         const size_t n3 = c->n3, nc = c->ncell, B3 = 3 * n3;
         const int nn = c->nn, nr = c->nr, nz = c->nz;
         // √N checkpointing: states every S steps; a segment of S steps is replayed with its tape

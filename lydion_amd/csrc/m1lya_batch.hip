@@ -1,12 +1,12 @@
 // m1lya_batch.hip -- the mixed-precision, batched M1 Lya step (m1lya_batch_* in
 // include/m1lya.h): kernels and C ABI.
 //
-// A batch holds clouds of different sizes (N_r, N_Z >= 3, 3 <= N_ν <= 128).  Their
+// A batch holds clouds of different sizes (N_r, N_Z >= 3, 3 <= N_ν <= 256).  Their
 // state J, H_r, H_Z is stored in FP32, normalised per cloud by a scale J0.  One step
 // of every cloud is one launch of k_step plus two small reduction launches:
 //
 //   * one wavefront marches along r over a segment of one cell row; lanes hold the
-//     frequency bins (lane l owns bins lR .. lR+R-1, R = 1 for N_ν <= 64, else 2);
+//     frequency bins (lane l owns bins lR .. lR+R-1, R = 1 for N_ν <= 64, 2 up to 128, 4 up to 256);
 //   * FP32 flux sweep + M1 closure (src/AdvectJandH.jl:526-1390, src/M1Closure.jl):
 //     the r interface between consecutive cells of the segment is evaluated once
 //     and carried in registers, the two Z interfaces of each cell in place;
@@ -14,7 +14,7 @@
 //     factors (src/FreqFactors.jl:142-278; each bin's exponentials are evaluated
 //     once and passed to the neighbouring rows by lane shuffles) and the frequency
 //     solve of src/UpdateLyaJandH.jl:178-391 as a wavefront-parallel cyclic
-//     reduction: one local level over the lane's bin pair, then parallel cyclic
+//     reduction: local levels over the lane's bins (one for two, two for four), then parallel cyclic
 //     reduction across the 64 lanes;
 //   * H rescale and causality, FP32 store into the other state buffer;
 //   * Neumaier-compensated FP64 accumulation of M_F, L, f_esc and the active-bin
@@ -55,7 +55,7 @@ constexpr double kJeps = 1e-30;          // J floor in units of J0 (m1f::kJeps32
 constexpr double kHeps2 = 1e-200;        // |H|^2 floor of the cell-centred closure
 constexpr double kEpsActive = 1e-10;     // FindActiveBins threshold
 constexpr double kPmin = 1e-3, kPmax = 40.0;
-constexpr int kMaxBins = 128;
+constexpr int kMaxBins = 256;                    // four bins per lane
 constexpr int kNScal = 7;                        // MF, L, net, out, esc, ΔJ sum, non-finite count
 constexpr int kPartStride = 2 * kMaxBins + 8;    // e1[N_ν], e10[N_ν], scalars
 constexpr int kMaxWaves = 8;
@@ -288,9 +288,10 @@ __device__ inline void D_edd64(double Hr, double Hz, double J, double& Drr, doub
 }
 
 // Bins of one lane: R consecutive bins starting at lane*R.
+template <int NQ>
 struct LaneBins {
-    int kb[2];
-    bool val[2], act[2];
+    int kb[NQ];
+    bool val[NQ], act[NQ];
 };
 
 // ------------------------------------------------------------------------
@@ -346,31 +347,34 @@ __device__ __forceinline__ void aniso_row(const BinRec* __restrict__ bins, const
 
 // LDS of a k_step block of wpb wavefronts (dynamic, so that 4-wavefront blocks keep four
 // blocks per CU while 8-wavefront blocks take two):
-//   sbin  BinRec[kMaxBins]              the cloud's frequency-bin constants, read every cell
-//   sh    double[wpb][kPartStride]      per-wavefront active-bin energies and step sums
+//   sbin  BinRec[64 NQ]                 the cloud's frequency-bin constants, read every cell
+//   sh    double[wpb][2 (64 NQ) + 8]    per-wavefront active-bin energies and step sums
 //   sacc  double[wpb][2 kNScal]         per-wavefront compensated sums (s, c)
-//   zrow  float[wpb + 4][3][kMaxBins]   Z stencil rows j0-2 .. j0+wpb+1: J, H_r/J, H_Z/J
-//   zflx  float[wpb + 1][3][kMaxBins]   Z interfaces j0 .. j0+wpb: F_J, F_Hr, F_HZ
+//   zrow  float[wpb + 4][3][64 NQ]      Z stencil rows j0-2 .. j0+wpb+1: J, H_r/J, H_Z/J
+//   zflx  float[wpb + 1][3][64 NQ]      Z interfaces j0 .. j0+wpb: F_J, F_Hr, F_HZ
+// (NQ: bins per lane of the instantiation, 2 for N_ν <= 128, 4 for N_ν <= 256)
 // (bin k of lane l at [q*64 + l] in zrow and zflx).
-constexpr size_t step_lds_bytes(int wpb)
+constexpr size_t step_lds_bytes(int wpb, int nq)
 {
-    return sizeof(BinRec) * kMaxBins + sizeof(double) * (size_t)wpb * (kPartStride + 2 * kNScal) +
-           sizeof(float) * 3 * kMaxBins * (size_t)(2 * wpb + 5);
+    return sizeof(BinRec) * 64 * nq + sizeof(double) * (size_t)wpb * ((2 * 64 * nq + 8) + 2 * kNScal) +
+           sizeof(float) * 3 * 64 * nq * (size_t)(2 * wpb + 5);
 }
 
-template <bool kShear>
-__global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+template <bool kShear, int NQ>
+__global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const Cloud* __restrict__ clouds, Arrays A,
                                                             const Seg* __restrict__ segs, int nblk, int wpb,
                                                             int parity, const int* __restrict__ krange,
                                                             double* __restrict__ part, const Ctrl* __restrict__ ctrl)
 {
     if (ctrl->halt) return;
+    constexpr int kB = 64 * NQ;          // bins a wavefront holds
+    constexpr int kSh = 2 * kB + 8;      // LDS row of the per-wavefront sums
     extern __shared__ __align__(16) unsigned char smem[];
     BinRec* sbin = reinterpret_cast<BinRec*>(smem);
-    double (*sh)[kPartStride] = reinterpret_cast<double (*)[kPartStride]>(sbin + kMaxBins);
+    double (*sh)[kSh] = reinterpret_cast<double (*)[kSh]>(sbin + kB);
     double (*sacc)[2 * kNScal] = reinterpret_cast<double (*)[2 * kNScal]>(&sh[wpb][0]);
-    float (*zrow)[3][kMaxBins] = reinterpret_cast<float (*)[3][kMaxBins]>(&sacc[wpb][0]);
-    float (*zflx)[3][kMaxBins] = zrow + (wpb + 4);
+    float (*zrow)[3][kB] = reinterpret_cast<float (*)[3][kB]>(&sacc[wpb][0]);
+    float (*zflx)[3][kB] = zrow + (wpb + 4);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63;
@@ -412,16 +416,16 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
     // lane index made opaque in every iteration: a compare costs one instruction there,
     // while masks hoisted out of the loop occupy SGPR pairs that spill to VGPR lanes.
     auto lane_bins = [&](int ln) {
-        LaneBins B;
+        LaneBins<NQ> B;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             B.kb[q] = ln * R + q;
             B.val[q] = (q < R) && (B.kb[q] < nn);
             B.act[q] = B.val[q] && B.kb[q] >= kmin && B.kb[q] <= kmax;
         }
         return B;
     };
-    LaneBins L = lane_bins(lane);
+    LaneBins<NQ> L = lane_bins(lane);
 
     const int i0 = sg.i0;
 
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         row = min(row, nz + 1);
         auto& cp = cloud_now();
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (!L.act[q]) continue;
             float vj, vr, vz, fr, fz;
             ld_z(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, nz, i, L.kb[q], row, bcb, bct, vj, vr, vz);
@@ -445,9 +449,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
 
     // r window (J, H_r/J, H_Z/J) of cells (i-1, i, i+1, i+2); FL / kimR: flux and
     // K_ϕϕ of the left interface of cell i.
-    float wJ[2][4], wR[2][4], wZ[2][4], FL[2][3], kimR[2];
+    float wJ[NQ][4], wR[NQ][4], wZ[NQ][4], FL[NQ][3], kimR[NQ];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < NQ; ++q) {
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             wJ[q][m] = 0.f; wR[q][m] = 0.f; wZ[q][m] = 0.f;
@@ -479,14 +483,14 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         const size_t cell = (size_t)j * nr + i;   // used only for valid rows, i >= i0
         const size_t base = cell * nn;
 
-        float af_c[2] = {0.f, 0.f};   // α_F of cell i (the left side of its right r interface)
+        float af_c[NQ] = {};   // α_F of cell i (the left side of its right r interface)
         // ---- right r interface of cell i (FP32) ----
-        float FR[2][3], kip[2], kimN[2];
+        float FR[NQ][3], kip[NQ], kimN[NQ];
         {
         auto& cp = cloud_now();
         const float* aF = cp.saF;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             FR[q][0] = FR[q][1] = FR[q][2] = 0.f;
             kip[q] = 0.f; kimN[q] = 0.f;
             if (!row_valid || !L.val[q]) continue;
@@ -517,7 +521,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         }
         if (first) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
                 kimR[q] = kimN[q];
             }
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         // ---- Z interfaces through LDS ----
         if (row_valid) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 if (!L.act[q]) continue;
                 const int p = q * 64 + lane;
                 zrow[wave + 2][0][p] = wJ[q][1];
@@ -545,17 +549,20 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         // Wavefront w evaluates interface w.  The block's top interface (f = wpb) is split
         // between wavefronts 0 and 1 by bin parity when every lane holds two bins, so that no
         // wavefront evaluates two full interfaces before the barrier.
-        const bool split_top = (R == 2) && (wpb >= 2);
+        // (with four bins per lane: the lower two to wavefront 0, the upper two to wavefront 1)
+        const bool split_top = (R >= 2) && (wpb >= 2);
+        constexpr int full = (1 << NQ) - 1;
+        const int lo = (R == 4) ? 0x3 : 0x1, hi = (R == 4) ? 0xC : 0x2;
 #pragma unroll 1
         for (int it = 0; it < 2; ++it) {
             const int f = it ? wpb : wave;
-            const int qm = !it ? 3 : (wave == 0) ? (split_top ? 1 : 3) : (wave == 1 && split_top) ? 2 : 0;
+            const int qm = !it ? full : (wave == 0) ? (split_top ? lo : full) : (wave == 1 && split_top) ? hi : 0;
             if (f > nvalid || qm == 0) continue;
             const int g = j0 + f;
             auto& cp = cloud_now();
             const float* aF = cp.saF;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 if (!L.act[q] || !((qm >> q) & 1)) continue;
                 const int k = L.kb[q];
                 const int p = q * 64 + lane;
@@ -581,10 +588,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
 
         auto& cv = cloud_now();
         const CellRec cr = cv.srec[cell];
-        double jo[2], hr[2], hz[2];
+        double jo[NQ], hr[NQ], hz[NQ];
         // ---- cell update (FP32): src/AdvectJandH.jl:1319-1379 ----
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
 #pragma clang fp contract(fast)
             jo[q] = 0.0; hr[q] = 0.0; hz[q] = 0.0;
             if (!L.val[q]) continue;
@@ -626,10 +633,12 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         const double al_pd = cr.alpha * cr.pd, src = cr.js * cv.src_scale;
         const float* Hvp = cv.sHv;
         const float* php = cv.sphi;
-        double hv[2] = {1.0, 1.0};
+        double hv[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) hv[q] = 1.0;
         double sr = 0.0, sz = 0.0;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (!L.val[q]) continue;
             const size_t idx = base + L.kb[q];
             const double h = Hvp[idx];
@@ -687,11 +696,11 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         const double PREF = 0.5 * (cr.alpha > cv.alpha_floor ? cr.alpha : cv.alpha_floor) * (cr.dnuD * cr.dnuD);
         const double R0 = cv.recoil ? kHoverKB / cr.T : 0.0;
         const int R1 = R - 1;
-        double ra[2], rb[2], rc[2], rd[2];
+        double ra[NQ], rb[NQ], rc[NQ], rd[NQ];
         if (!aniso) {
-            double Aq[2], oLq[2], Bpq[2], oRpq[2];
+            double Aq[NQ], oLq[NQ], Bpq[NQ], oRpq[NQ];
     #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 Aq[q] = 1.0; oLq[q] = 1.0; Bpq[q] = 0.0; oRpq[q] = 1.0;
                 if (!L.val[q]) continue;
                 const BinRec bk = bins[L.kb[q]];
@@ -733,23 +742,30 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
                 oRpq[q] = oRp;
             }
             // neighbours: bin k-1 -> oL (lane above for q = 0), bin k+1 -> B', oR' (lane below for q = R-1)
-            const double oL_prev = from_prev_lane(R1 ? oLq[1] : oLq[0]);
+            auto lane_last = [&](const double(&v)[NQ]) {   // the value of the lane's last bin, q = R-1
+                double r = v[0];
+#pragma unroll
+                for (int q = 1; q < NQ; ++q) r = (q == R1) ? v[q] : r;
+                return r;
+            };
+            const double oL_prev = from_prev_lane(lane_last(oLq));
             const double Bp_next = from_next_lane(Bpq[0]), oRp_next = from_next_lane(oRpq[0]);
             // interfaces owned by this lane: Gq[q] = G at the top of bin lR+q
-            double Gq[2];
-            Gq[0] = drcp0(Aq[0] - ((R1 == 0) ? Bp_next : Bpq[1]));
-            Gq[1] = drcp0(Aq[1] - Bp_next);
-            const double G_prev = from_prev_lane(R1 ? Gq[1] : Gq[0]);
+            double Gq[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                Gq[q] = drcp0(Aq[q] - ((q == R1) ? Bp_next : Bpq[q + 1 < NQ ? q + 1 : NQ - 1]));
+            const double G_prev = from_prev_lane(lane_last(Gq));
 
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
                 const int k = L.kb[q];
                 if (!L.val[q] || k < 1 || k > nn - 2) continue;
                 const double idn = bins[k].inv_dnu;
-                const double oLm = (q == 0) ? oL_prev : oLq[0];
-                const double oRpp = (q == R1) ? oRp_next : oRpq[1];
-                const double Gm = (q == 0) ? G_prev : Gq[0];
+                const double oLm = (q == 0) ? oL_prev : oLq[q > 0 ? q - 1 : 0];
+                const double oRpp = (q == R1) ? oRp_next : oRpq[q + 1 < NQ ? q + 1 : NQ - 1];
+                const double Gm = (q == 0) ? G_prev : Gq[q > 0 ? q - 1 : 0];
                 const double Gp = Gq[q];
                 const double Fkp = oRpp * Gp * idn;
                 const double Fkm = oLm * Gm * idn;
@@ -765,7 +781,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             // one bin at a time (a scheduling barrier between them keeps the two rows' temporaries
             // from being live together)
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 ra[q] = 0.0; rb[q] = 1.0; rc[q] = 0.0; rd[q] = L.val[q] ? jo[q] : 0.0;
                 const int k = L.kb[q];
                 if (L.val[q] && k >= 1 && k <= nn - 2)
@@ -775,10 +791,30 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             }
         }
 
-        // ================= tridiagonal solve: CR level + PCR over the lanes =================
+        // ================= tridiagonal solve: CR levels + PCR over the lanes =================
         double a, b, c, d, ua = 0.0, uc = 0.0, ud = 0.0;
-        if (R == 2) {
-            // eliminate the odd bin of each lane's pair into the even rows
+        // four bins per lane: a first cyclic-reduction level eliminates bins 1 and 3 of the lane into
+        // bins 0 and 2 (bin 3 of the lane above couples to bin 0); rows 0' and 2' then form the
+        // two-bins-per-lane system the level below reduces
+        double u1a = 0.0, u1c = 0.0, u1d = 0.0, u3a = 0.0, u3c = 0.0, u3d = 0.0;
+        if (NQ == 4 && R == 4) {
+            const double ib1 = drcp1(rb[1]), ib3 = drcp1(rb[NQ - 1]);
+            u1a = ra[1] * ib1; u1c = rc[1] * ib1; u1d = rd[1] * ib1;
+            u3a = ra[NQ - 1] * ib3; u3c = rc[NQ - 1] * ib3; u3d = rd[NQ - 1] * ib3;
+            const double pa = from_prev_lane(u3a), pc = from_prev_lane(u3c), pd3 = from_prev_lane(u3d);
+            const double b0 = rb[0] - ra[0] * pc - rc[0] * u1a;
+            const double d0 = rd[0] - ra[0] * pd3 - rc[0] * u1d;
+            const double a0 = -ra[0] * pa;
+            const double c0 = -rc[0] * u1c;
+            const double b2 = rb[2 % NQ] - ra[2 % NQ] * u1c - rc[2 % NQ] * u3a;
+            const double d2 = rd[2 % NQ] - ra[2 % NQ] * u1d - rc[2 % NQ] * u3d;
+            const double a2 = -ra[2 % NQ] * u1a;
+            const double c2 = -rc[2 % NQ] * u3c;
+            ra[0] = a0; rb[0] = b0; rc[0] = c0; rd[0] = d0;
+            ra[1] = a2; rb[1] = b2; rc[1] = c2; rd[1] = d2;
+        }
+        if (R >= 2) {
+            // eliminate the odd row of each lane's pair into the even rows
             const double ib1 = drcp1(rb[1]);
             ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
             const double pua = from_prev_lane(ua), puc = from_prev_lane(uc), pud = from_prev_lane(ud);
@@ -817,11 +853,20 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
             b = nb;
             d = nd;
         }
-        double xs[2];
+        double xs[NQ];
         xs[0] = d * drcp1(b);
-        xs[1] = 0.0;
-        if (R == 2) {
-            xs[1] = ud - ua * xs[0] - uc * from_next_lane(xs[0]);
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) xs[q] = 0.0;
+        if (R >= 2) {
+            const double x0n = from_next_lane(xs[0]);
+            xs[1] = ud - ua * xs[0] - uc * x0n;
+            if (NQ == 4 && R == 4) {
+                // back through the first level: bins 2, 1, 3 of the lane from bins 0, 2 and the next lane's 0
+                const double x2 = xs[1];
+                xs[2 % NQ] = x2;
+                xs[1] = u1d - u1a * xs[0] - u1c * x2;
+                xs[NQ - 1] = u3d - u3a * x2 - u3c * x0n;
+            }
         }
 
         // ================= H correction, store, reductions =================
@@ -829,7 +874,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, 4) void k_step(const Cloud* __restr
         const double Azb = (bcb == M1LYA_OUTFLOW) ? cr.A_zbot : 0.0;
         double escl = 0.0;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (!L.val[q]) continue;
             const int k = L.kb[q];
             // J floor, H rescale and causality: src/UpdateLyaJandH.jl:335-379
@@ -900,14 +945,14 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_active_rows(const Cloud* __r
     const Seg sg = segs[(size_t)lb * wpb + wave];
     const Cloud& cb = clouds[segs[(size_t)lb * wpb].cloud];
     const int nnb = cb.nn, R = cb.R;
-    double e1[2] = {0.0, 0.0}, e10[2] = {0.0, 0.0};
+    double e1[4] = {0.0, 0.0, 0.0, 0.0}, e10[4] = {0.0, 0.0, 0.0, 0.0};
     if (sg.j < cb.nz) {
         const float* J = A.J[parity] + cb.o3;
         for (int i = sg.i0; i < sg.i1; ++i) {
             const size_t cell = (size_t)sg.j * cb.nr + i;
             const double dV = A.rec[cb.oc + cell].dV;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < 4; ++q) {
                 const int k = lane * R + q;
                 if (q >= R || k >= nnb) continue;
                 const double Jd = J[cell * nnb + k];
@@ -918,7 +963,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_active_rows(const Cloud* __r
         }
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
         const int k = lane * R + q;
         if (q < R && k < nnb) {
             sh[wave][k] = e1[q] * cb.J0;
@@ -1097,6 +1142,7 @@ struct m1lya_batch {
     double* rscratch = nullptr;            // sized for the largest cloud
     double nu_lya = 0.0;
     bool shear = false;                    // a cell's velocity gradient is sheared or anisotropic
+    bool wide = false;                     // a cloud has more than 128 bins (four per lane)
     std::vector<BinRec> h_bin;
     CellRec* d_rec = nullptr;
     BinRec* d_bin = nullptr;
@@ -1339,13 +1385,23 @@ void launch_step_kernels(m1lya_batch* b, int parity)
         BProf p(b, "k_step", true);
         // the variant with the reference-row path for sheared / anisotropic velocity gradients only
         // when a cell of the batch needs it: it costs registers the common path does not pay for
-        const size_t lds = step_lds_bytes(b->wpb);
-        if (b->shear)
-            hipLaunchKernelGGL(k_step<true>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
-                               b->nblk, b->wpb, parity, b->krange, b->part, b->ctrl);
-        else
-            hipLaunchKernelGGL(k_step<false>, dim3(b->grid), dim3(64 * b->wpb), lds, s, b->d_cl, b->A, b->d_segs,
-                               b->nblk, b->wpb, parity, b->krange, b->part, b->ctrl);
+        // and the four-bins-per-lane variant only when a cloud has more than 128 bins
+        const size_t lds = step_lds_bytes(b->wpb, b->wide ? 4 : 2);
+        const dim3 grid(b->grid), block(64 * b->wpb);
+        if (b->wide) {
+            if (b->shear)
+                hipLaunchKernelGGL((k_step<true, 4>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
+                                   parity, b->krange, b->part, b->ctrl);
+            else
+                hipLaunchKernelGGL((k_step<false, 4>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk,
+                                   b->wpb, parity, b->krange, b->part, b->ctrl);
+        } else if (b->shear) {
+            hipLaunchKernelGGL((k_step<true, 2>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
+                               parity, b->krange, b->part, b->ctrl);
+        } else {
+            hipLaunchKernelGGL((k_step<false, 2>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
+                               parity, b->krange, b->part, b->ctrl);
+        }
         p.done();
     }
     {
@@ -1515,7 +1571,7 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
     for (int c = 0; c < n_clouds; ++c)
         if (n_nu[c] < 3 || n_nu[c] > kMaxBins || n_r[c] < 3 || n_z[c] < 3) {
             g_batch_error = "m1lya_batch_create: cloud " + std::to_string(c) +
-                            " needs 3 <= N_nu <= 128 and N_r, N_Z >= 3";
+                            " needs 3 <= N_nu <= " + std::to_string(kMaxBins) + " and N_r, N_Z >= 3";
             return M1LYA_ERR_DIMS;
         }
     m1lya_batch* b = new m1lya_batch();
@@ -1530,7 +1586,8 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
             Cloud& cl = b->cl[c];
             std::memset(&cl, 0, sizeof(Cloud));
             cl.nn = n_nu[c]; cl.nr = n_r[c]; cl.nz = n_z[c];
-            cl.R = cl.nn <= 64 ? 1 : 2;
+            cl.R = cl.nn <= 64 ? 1 : cl.nn <= 128 ? 2 : 4;
+            if (cl.R == 4) b->wide = true;
             const size_t ncell = (size_t)cl.nr * cl.nz;
             cl.o3 = (long long)b->n3; b->n3 += ncell * cl.nn;
             cl.oc = (long long)b->nc; b->nc += ncell;
@@ -2014,6 +2071,76 @@ int m1lya_batch_set_graph(m1lya_batch* b, int on)
 // Times every tile shape of the candidate set on the live batch and keeps the
 // fastest.  State, active-bin ranges and buffer parity are restored, so the call
 // does not advance the simulation (the step scalars are those of the last trial).
+// Everything m1lya_batch_autotune changes, put back on every exit: the trial buffers and
+// events are released, profiling is restored, and unless the sweep finished the tile shape,
+// the saved state and the buffer parity return to what they were before the call.
+struct AutotuneScope {
+    m1lya_batch* b;
+    size_t bytes = 0;
+    float* sJ = nullptr;
+    float* sR = nullptr;
+    float* sZ = nullptr;
+    int* sK = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int prof, seg, wpb, parity;
+    bool saved = false, finished = false;
+
+    explicit AutotuneScope(m1lya_batch* bb)
+        : b(bb), prof(bb->profiling), seg(bb->seg_len), wpb(bb->wpb), parity(bb->parity) {}
+
+    void copy(void* dst, const void* src, size_t n)
+    {
+        bchk(b, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream), "autotune copy");
+    }
+    void save()
+    {
+        bytes = b->n3 * sizeof(float);
+        sJ = balloc<float>(b, b->n3);
+        sR = balloc<float>(b, b->n3);
+        sZ = balloc<float>(b, b->n3);
+        sK = balloc<int>(b, (size_t)b->n * 2);
+        copy(sJ, b->A.J[parity], bytes);
+        copy(sR, b->A.Hr[parity], bytes);
+        copy(sZ, b->A.Hz[parity], bytes);
+        copy(sK, b->krange, b->n * 2 * sizeof(int));
+        saved = true;
+    }
+    void restore()
+    {
+        copy(b->A.J[parity], sJ, bytes);
+        copy(b->A.Hr[parity], sR, bytes);
+        copy(b->A.Hz[parity], sZ, bytes);
+        copy(b->krange, sK, b->n * 2 * sizeof(int));
+        b->parity = parity;
+    }
+    ~AutotuneScope()
+    {
+        if (!finished) {
+            // an error left a trial candidate in place: best effort back to the entry state,
+            // the error itself is what the call reports
+            try {
+                if (b->seg_len != seg || b->wpb != wpb) {
+                    b->seg_len = seg;
+                    b->wpb = wpb;
+                    build_tiles(b);
+                    sync_clouds(b);
+                }
+                if (saved) restore();
+            } catch (...) {
+            }
+            b->parity = parity;
+        }
+        (void)hipStreamSynchronize(b->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        bfree(b, sJ);
+        bfree(b, sR);
+        bfree(b, sZ);
+        bfree(b, sK);
+        b->profiling = prof;
+    }
+};
+
 int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* waves_per_block, double* best_ms)
 {
     return bguard(b, [&] {
@@ -2023,27 +2150,8 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         sync_clouds(b);
         if (!b->krange_valid) compute_krange(b);
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        const int p0 = b->parity;
-        const size_t bytes = b->n3 * sizeof(float);
-        float* sJ = balloc<float>(b, b->n3);
-        float* sR = balloc<float>(b, b->n3);
-        float* sZ = balloc<float>(b, b->n3);
-        int* sK = balloc<int>(b, (size_t)b->n * 2);
-        auto copy = [&](void* dst, const void* src, size_t n) {
-            bchk(b, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream), "autotune copy");
-        };
-        copy(sJ, b->A.J[p0], bytes);
-        copy(sR, b->A.Hr[p0], bytes);
-        copy(sZ, b->A.Hz[p0], bytes);
-        copy(sK, b->krange, b->n * 2 * sizeof(int));
-        auto restore = [&] {
-            copy(b->A.J[p0], sJ, bytes);
-            copy(b->A.Hr[p0], sR, bytes);
-            copy(b->A.Hz[p0], sZ, bytes);
-            copy(b->krange, sK, b->n * 2 * sizeof(int));
-            b->parity = p0;
-        };
-        const int prof = b->profiling;
+        AutotuneScope scope(b);
+        scope.save();
         b->profiling = 0;
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         set_ctrl(b, nullptr, 0);
@@ -2055,9 +2163,8 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
         static const int wpb_opts[] = {2, 4, 8};
         int best_s = b->seg_len, best_w = b->wpb;
         double best = 1e300;
-        hipEvent_t e0, e1;
-        bchk(b, hipEventCreate(&e0), "hipEventCreate");
-        bchk(b, hipEventCreate(&e1), "hipEventCreate");
+        bchk(b, hipEventCreate(&scope.e0), "hipEventCreate");
+        bchk(b, hipEventCreate(&scope.e1), "hipEventCreate");
         // two passes over the candidates, each candidate keeping its faster pass: the first
         // candidates of a single pass ran while the clocks were still rising
         for (int pass = 0; pass < 2; ++pass)
@@ -2067,18 +2174,18 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
                 b->wpb = w;
                 build_tiles(b);
                 sync_clouds(b);
-                restore();
+                scope.restore();
                 launch_step_kernels(b, b->parity);   // warm-up
                 b->parity ^= 1;
-                bchk(b, hipEventRecord(e0, b->stream), "hipEventRecord");
+                bchk(b, hipEventRecord(scope.e0, b->stream), "hipEventRecord");
                 for (int t = 0; t < trial_steps; ++t) {
                     launch_step_kernels(b, b->parity);
                     b->parity ^= 1;
                 }
-                bchk(b, hipEventRecord(e1, b->stream), "hipEventRecord");
-                bchk(b, hipEventSynchronize(e1), "hipEventSynchronize");
+                bchk(b, hipEventRecord(scope.e1, b->stream), "hipEventRecord");
+                bchk(b, hipEventSynchronize(scope.e1), "hipEventSynchronize");
                 float ms = 0.f;
-                bchk(b, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                bchk(b, hipEventElapsedTime(&ms, scope.e0, scope.e1), "hipEventElapsedTime");
                 const double per = ms / trial_steps;
                 if (per < best) {
                     best = per;
@@ -2086,19 +2193,13 @@ int m1lya_batch_autotune(m1lya_batch* b, int trial_steps, int* seg_len, int* wav
                     best_w = w;
                 }
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         b->seg_len = best_s;
         b->wpb = best_w;
         build_tiles(b);
         sync_clouds(b);
-        restore();
+        scope.restore();
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        bfree(b, sJ);
-        bfree(b, sR);
-        bfree(b, sZ);
-        bfree(b, sK);
-        b->profiling = prof;
+        scope.finished = true;
         if (seg_len) *seg_len = best_s;
         if (waves_per_block) *waves_per_block = best_w;
         if (best_ms) *best_ms = best;

@@ -9,6 +9,9 @@ namespace m1 {
 
 // Cell columns per wavefront in the column kernel (lane-per-column dgtsv).
 constexpr int kColumnW = 4;
+// Largest N_ν of the FP64 context (nine bins per lane, one column per wavefront).
+constexpr int kMaxBinsFP64 = 576;
+constexpr int kMaxBinsAdjoint = 256;
 
 // LDS row stride of one column's tridiagonal arrays: >= N_ν - 2 rows, odd.
 __host__ __device__ inline int col_stride(int nn)
@@ -234,7 +237,8 @@ void launch_reduce_cols(hipStream_t s, const double* part, int nrows, int stride
                         const int* halt = nullptr);
 void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, int kcmax, int* krange);
 void launch_advect(hipStream_t s, const AdvectArgs& a);
-int column_width();
+// Cell columns per wavefront of the column kernel for an N_ν-bin grid.
+int column_width(int nn);
 // Blocks of the column kernel for ngroups column groups: a fixed count (at most ngroups),
 // so that the partition of the fused reductions, and with it their bits, does not depend
 // on the device.  2048 is one launch wave on MI355X (8 blocks per CU, 256 CUs).

@@ -819,7 +819,10 @@ static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
     hipLaunchKernelGGL((k_column<W, KPL>), dim3(nblocks), dim3(64), lds, s, a);
 }
 
-int column_width() { return kColumnW; }
+// Up to 256 bins (four per lane) four cell columns share a wavefront; beyond, up to 576 bins
+// (nine per lane, the reference's q = 1.03 grid of 542 bins), one column per wavefront keeps the
+// per-lane registers and the LDS rows within bounds.
+int column_width(int nn) { return nn <= 256 ? kColumnW : 1; }
 
 // A grid of exactly one wave of resident blocks: every block loops over the same number of
 // column groups and they all start together.  A larger grid left a partial second wave of
@@ -838,7 +841,12 @@ void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks)
     case 1: launch_column_t<kColumnW, 1>(s, a, nblocks); break;
     case 2: launch_column_t<kColumnW, 2>(s, a, nblocks); break;
     case 3: launch_column_t<kColumnW, 3>(s, a, nblocks); break;
-    default: launch_column_t<kColumnW, 4>(s, a, nblocks); break;
+    case 4: launch_column_t<kColumnW, 4>(s, a, nblocks); break;
+    case 5: launch_column_t<1, 5>(s, a, nblocks); break;
+    case 6: launch_column_t<1, 6>(s, a, nblocks); break;
+    case 7: launch_column_t<1, 7>(s, a, nblocks); break;
+    case 8: launch_column_t<1, 8>(s, a, nblocks); break;
+    default: launch_column_t<1, 9>(s, a, nblocks); break;
     }
 }
 
@@ -1157,7 +1165,12 @@ void launch_absorb(hipStream_t s, const AbsorbArgs& a)
     case 1: hipLaunchKernelGGL(k_absorb<1>, dim3(a.ncell), dim3(64), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_absorb<2>, dim3(a.ncell), dim3(64), 0, s, a); break;
     case 3: hipLaunchKernelGGL(k_absorb<3>, dim3(a.ncell), dim3(64), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_absorb<4>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_absorb<4>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_absorb<5>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_absorb<6>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(k_absorb<7>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_absorb<8>, dim3(a.ncell), dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_absorb<9>, dim3(a.ncell), dim3(64), 0, s, a); break;
     }
 }
 

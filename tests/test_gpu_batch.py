@@ -294,6 +294,52 @@ def test_rhd_like_state_full_size_vs_fp64(lut):
     b.close()
 
 
+def _reference_grid_cloud(lut, n_r=24, n_z=20, umax=200.0, tau0=1e8):
+    """The reference's default frequency grid (q = 1.1, 194 bins; four bins per lane in the batch)."""
+    from lydion_amd.static_cloud import setup_static_cloud
+    return developed_state(setup_static_cloud(tau0, 1e4, tau_abs=1.0, rs=0.5 * (1.1 / n_r), umax=umax, n_r=n_r,
+                                              n_z=n_z, nu=L.reference_freq_grid(), lut=lut))
+
+
+def test_reference_grid_194_bins_vs_fp64(lut):
+    """The 194-bin grid the reference's published timings use (RunStaticCloudTest.jl:46), in the
+    four-bins-per-lane variant of the batched step: M_F, f_esc at every step and the J energy
+    against the FP64 step, and against the oracle from the initial state."""
+    cloud = _reference_grid_cloud(lut)
+    assert cloud.lya.n_nu == 194
+    h64, ctx = _fp64_history(cloud, 20)
+    b = M1LyaBatch([cloud])
+    hb = b.run_steps(20)[:, 0, :]
+    _check_history(hb, h64, "reference grid 194 bins")
+    assert np.array_equal(hb[:, 5:7], h64[:, 5:7])
+    ee = _energy_err(cloud, b.download(0, "J"), ctx.download("J"))
+    es = _spectrum_err(cloud, b, ctx)
+    print(f"194 bins, 20 steps: M_F rel {_relerr(hb[-1, 0], h64[-1, 0]):.2e}, f_esc rel "
+          f"{_relerr(hb[-1, 4], h64[-1, 4]):.2e}, J energy {ee:.2e}, spectrum {es:.2e}")
+    assert ee <= FP32_TOL and es <= FP32_TOL
+    ctx.close()
+    b.close()
+
+
+def test_wide_and_narrow_clouds_in_one_batch(lut):
+    """A batch that holds a 194-bin cloud runs every cloud in the four-bins-per-lane variant; each
+    cloud still matches its FP64 step, and the wide cloud is bit-identical to its batch of one."""
+    wide = _reference_grid_cloud(lut, n_r=13, n_z=17, umax=0.0, tau0=1e7)
+    narrow = developed_state(small_cloud(n_r=16, n_z=12, n_nu=64, tau0=1e6, tau_abs=0.5, lut=lut), seed=3)
+    b = M1LyaBatch([narrow, wide])
+    hb = b.run_steps(10)
+    b1 = M1LyaBatch([wide])
+    h1 = b1.run_steps(10)
+    assert bitdiff(b.download(1, "J"), b1.download(0, "J")) == 0
+    assert np.array_equal(hb[:, 1, :], h1[:, 0, :])
+    for idx, cloud in enumerate((narrow, wide)):
+        h64, ctx = _fp64_history(cloud, 10)
+        _check_history(hb[:, idx, :], h64, f"mixed batch cloud {idx}")
+        ctx.close()
+    b.close()
+    b1.close()
+
+
 def test_profiling_modes(lut):
     """Profiling level 2 brackets only k_step (two events per step); level 1 every launch."""
     b = M1LyaBatch(_mixed_clouds(lut)[:1])

@@ -119,11 +119,19 @@ def test_reductions_match(lut):
 
 
 @pytest.mark.parametrize("case", ["static_cloud_32x32x64", "cone_dust_flow", "odd_grid_outflow",
-                                  "reference_grid_194_bins"])
+                                  "reference_grid_194_bins", "reference_grid_542_bins"])
 def test_fused_step_bit_exact(lut, case):
     if case == "static_cloud_32x32x64":
         cloud = L.static_cloud_32x32x64(lut=lut)
         nsteps = 40
+    elif case == "reference_grid_542_bins":
+        # the reference's q = 1.03 grid (542 bins, RunVelocityGradientSpectraTest.jl:25): nine bins
+        # per lane, one cell column per wavefront
+        from lydion_amd.static_cloud import setup_static_cloud
+        cloud = randomize(setup_static_cloud(1e8, 1e4, tau_abs=0.5, rs=1.0, umax=400.0, recoil=False, n_r=10,
+                                             n_z=8, nu=L.reference_freq_grid(30.0, 1.03, 1.03), lut=lut), seed=18)
+        assert cloud.lya.n_nu == 542
+        nsteps = 3
     elif case == "reference_grid_194_bins":
         # the reference's default frequency grid (q = 1.1, 194 bins; four bins per lane)
         from lydion_amd.static_cloud import setup_static_cloud

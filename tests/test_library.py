@@ -40,7 +40,7 @@ def test_bad_arguments_fail_loudly_without_a_device():
     h = C.c_void_p()
     assert lib.m1lya_create(2, 8, 8, 0, C.byref(h)) == _lib.ERR_DIMS
     assert lib.m1lya_create(64, 2, 8, 0, C.byref(h)) == _lib.ERR_DIMS
-    assert lib.m1lya_create(512, 8, 8, 0, C.byref(h)) == _lib.ERR_DIMS
+    assert lib.m1lya_create(577, 8, 8, 0, C.byref(h)) == _lib.ERR_DIMS
     buf = C.create_string_buffer(256)
     lib.m1lya_last_error(None, buf, 256)
     assert b"N_" in buf.value
@@ -54,7 +54,7 @@ def test_batch_bad_arguments_fail_loudly_without_a_device():
     h = C.c_void_p()
     ints = lambda *v: (C.c_int * len(v))(*v)
     # dimension checks run before any HIP call
-    assert lib.m1lya_batch_create(2, ints(64, 129), ints(8, 8), ints(8, 8), C.byref(h)) == _lib.ERR_DIMS
+    assert lib.m1lya_batch_create(2, ints(64, 257), ints(8, 8), ints(8, 8), C.byref(h)) == _lib.ERR_DIMS
     assert lib.m1lya_batch_create(1, ints(64), ints(2), ints(8), C.byref(h)) == _lib.ERR_DIMS
     assert lib.m1lya_batch_create(0, ints(64), ints(8), ints(8), C.byref(h)) == _lib.ERR_ARG
     buf = C.create_string_buffer(256)
