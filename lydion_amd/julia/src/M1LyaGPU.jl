@@ -20,7 +20,7 @@ export attach!, download_state!, upload_state!, edge_spectra, AdvectLyα!, LyαE
        FrequencyUpdateJLyα_AllCells!, UpdateFreqFactors!, Update_f_Lyα_MF_L_Lyα!,
        Compute_f_esc_Lyα_Cylindrical, Reset_Δρu!, Update_e_Lyα!, BatchM1Lya, run_batch!, adjoint_run!,
        set_voigt_lut!, prepare_subcycling!, set_adjoint_mode!, run_until_steady!, GetRHDCFLTimeStep,
-       FindRSL_LyαRT, absorption_substep!, run_MF
+       FindRSL_LyαRT, absorption_substep!, run_MF, prepare_subcycling_chem!, run_batch_until_steady!
 
 const libm1lya = joinpath(@__DIR__, "..", "..", "libm1lya.so")
 
@@ -307,6 +307,18 @@ function run_batch!(b::Ptr{Cvoid}, nsteps::Integer)
     return hist
 end
 
+"""Steps every cloud until each one's ΔJ/J < tol, or max_steps; returns the [8, nclouds, steps]
+step scalars (row 8 is ΔJ/J) and, per cloud, the step after which it was first below tol (0: never)."""
+function run_batch_until_steady!(b::Ptr{Cvoid}, max_steps::Integer, tol::Float64)
+    n = ccall((:m1lya_batch_count, libm1lya), Cint, (Ptr{Cvoid},), b)
+    hist = Array{Float64}(undef, 8, n, max(max_steps, 1))
+    steady = zeros(Cint, n)
+    done = Ref{Cint}(0)
+    bcheck(ccall((:m1lya_batch_run_until_steady, libm1lya), Cint,
+                 (Ptr{Cvoid}, Cint, Float64, Ptr{Float64}, Ptr{Cint}, Ref{Cint}), b, max_steps, tol, hist, steady, done), b)
+    return hist[:, :, 1:done[]], steady
+end
+
 # --- the M_F of each of n fused steps: the primal whose rrule (ext/M1LyaGPUChainRulesCoreExt.jl)
 #     replays the steps through m1lya_adjoint_run ---
 function run_MF(Lyα, nsteps::Integer, c̃::Float64, Δt::Float64, Z_center::Float64; BC_Z_bot::Symbol=:Reflect)
@@ -384,6 +396,21 @@ function prepare_subcycling!(Lyα, T::Matrix{Float64}, nHI::Matrix{Float64}; αc
     GC.@preserve T nHI αc αc_F js pd check(ccall((:m1lya_prepare_subcycling, libm1lya), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
         ctx, T, nHI, optr(αc), optr(αc_F), optr(js), optr(pd), Cint(all_bins)), ctx)
+    return nothing
+end
+
+# --- PrepareLyαSubCycling! from the chemical state (PhotoThermoChemistry.jl:2086-2155): ∫dν js of
+#     recombinations and collisions, p_d, the dust and gas continuum opacities, then the refresh above ---
+"""`chem` is [N_r, N_Z, 8] = T, n_HI, n_H2, n_HII, n_HeI, n_e, n_OHx, n_CHx; `ρd` [3, N_r, N_Z] (or
+nothing); `κ` = (κd_abs_Lyα, κd_F_Lyα) of silicate, graphite, PAH.  Needs `set_voigt_lut!`."""
+function prepare_subcycling_chem!(Lyα, chem::Array{Float64,3}, ρd, κ::AbstractVector{Float64}; z::Float64=0.0,
+                                  ots::Bool=true, destruction::Bool=true, all_bins::Bool=true)
+    ctx = ctxof(Lyα)
+    length(κ) == 6 || throw(ArgumentError("κ holds the 6 dust opacities"))
+    κv = collect(κ)
+    GC.@preserve chem ρd κv check(ccall((:m1lya_prepare_subcycling_chem, libm1lya), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Cint, Cint, Cint),
+        ctx, chem, optr(ρd), κv, z, Cint(ots), Cint(destruction), Cint(all_bins)), ctx)
     return nothing
 end
 

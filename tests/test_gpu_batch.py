@@ -323,21 +323,24 @@ def test_reference_grid_194_bins_vs_fp64(lut):
 
 def test_wide_and_narrow_clouds_in_one_batch(lut):
     """A batch that holds a 194-bin cloud runs every cloud in the four-bins-per-lane variant; each
-    cloud still matches its FP64 step, and the wide cloud is bit-identical to its batch of one."""
+    cloud still matches its FP64 step, and each is bit-identical to its batch of one (the narrow
+    one run there by the two-bins-per-lane variant)."""
     wide = _reference_grid_cloud(lut, n_r=13, n_z=17, umax=0.0, tau0=1e7)
     narrow = developed_state(small_cloud(n_r=16, n_z=12, n_nu=64, tau0=1e6, tau_abs=0.5, lut=lut), seed=3)
     b = M1LyaBatch([narrow, wide])
     hb = b.run_steps(10)
-    b1 = M1LyaBatch([wide])
-    h1 = b1.run_steps(10)
-    assert bitdiff(b.download(1, "J"), b1.download(0, "J")) == 0
-    assert np.array_equal(hb[:, 1, :], h1[:, 0, :])
+    for idx, cloud in enumerate((narrow, wide)):
+        b1 = M1LyaBatch([cloud])
+        h1 = b1.run_steps(10)
+        for key in ("J", "H_r", "H_z"):
+            assert bitdiff(b.download(idx, key), b1.download(0, key)) == 0, (idx, key)
+        assert np.array_equal(hb[:, idx, :], h1[:, 0, :]), idx
+        b1.close()
     for idx, cloud in enumerate((narrow, wide)):
         h64, ctx = _fp64_history(cloud, 10)
         _check_history(hb[:, idx, :], h64, f"mixed batch cloud {idx}")
         ctx.close()
     b.close()
-    b1.close()
 
 
 def test_profiling_modes(lut):
