@@ -401,7 +401,7 @@ int  m1lya_batch_run_until_steady(m1lya_batch *b, int max_steps, double tol, dou
 int  m1lya_batch_scalars(m1lya_batch *b, double *out);
 /* [n_clouds][2] active-bin ranges the next step uses (0-based, inclusive). */
 int  m1lya_batch_active_bins(m1lya_batch *b, int *out);
-/* Tile shape: cells per wavefront segment along r, wavefronts per block (1..8).
+/* Tile shape: cells per wavefront segment along r (1..64), wavefronts per block (1..8).
  * The state after a step does not depend on the tile shape. */
 int  m1lya_batch_set_tiles(m1lya_batch *b, int seg_len, int waves_per_block);
 int  m1lya_batch_get_tiles(const m1lya_batch *b, int *seg_len, int *waves_per_block);

@@ -155,6 +155,14 @@ __device__ __forceinline__ double dpp_f64(double v)
 __device__ __forceinline__ double from_prev_lane(double v) { return dpp_f64<0x138>(v); }   // 0 in lane 0
 __device__ __forceinline__ double from_next_lane(double v) { return dpp_f64<0x130>(v); }   // 0 in lane 63
 
+// The value of lane addr/4 (ds_bpermute, any lane of the wavefront).
+__device__ __forceinline__ double bperm_f64(int addr, double v)
+{
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double lane_value(double v, int l)
 {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
@@ -472,6 +480,8 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
     // per lane: Σ Δν ΔV |J_new - J_start| (the cloud's ΔJ/J) and the non-finite outputs
     double conv = 0.0;
     int bad = 0;
+    // lane t: Σ_k Δν (ΔH_r, ΔH_Z) of the emission/absorption update of cell i0 + t
+    double seg_sr = 0.0, seg_sz = 0.0;
 
     // All wavefronts of the block march over the same cells [i0, i1) in lockstep;
     // i = i0-1 only evaluates the left r interface of cell i0.
@@ -656,21 +666,11 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         }
         sr = wave_sum64(sr);
         sz = wave_sum64(sz);
-        {
-            // Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660)
-            const double fr = sr * cv.mf_scale, fz = sz * cv.mf_scale;
-            if (lane == 0) {
-                cloud_now().sfr[cell] = fr;
-                cloud_now().sfz[cell] = fz;
-            }
-            kadd(S_MF, cr.dV * sqrt(fr * fr + fz * fz));
-            kadd(S_L, kFourPi * cr.dV * cr.js);
-            const double ddr = cr.rcom, ddz = cr.zcom - cv.z_center;
-            const double d2 = ddr * ddr + ddz * ddz;
-            double frad = 0.0;
-            if (d2 > 0.0) frad = (fr * ddr + fz * ddz) * drcp1(sqrt(d2));
-            kadd(S_NET, cr.dV * frad);
-            kadd(S_OUT, cr.dV * (frad > 0.0 ? frad : 0.0));
+        // the cell's Σ Δν ΔH goes to lane i - i0: the per-cell terms of Update_f_Lyα_MF_L_Lyα!
+        // are formed after the march, one cell per lane, instead of by the whole wavefront
+        if (ln == i - i0) {
+            seg_sr = sr;
+            seg_sz = sz;
         }
 
         // ================= frequency factors (FP64) =================
@@ -838,14 +838,18 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             b = nb;
             d = nd;
         }
+        // Levels 2..32 read lanes l -/+ s through ds_bpermute, wrapping around the wavefront.
+        // The wrapped values meet zero coefficients: the lanes below s have a = 0 and those
+        // above 63 - s have c = 0 at level s (bin 0, bin N_ν-1 and the bins past N_ν are
+        // identity rows, and level s/2 zeroed the next s/2 lanes by multiplying with them),
+        // and the values are finite, so no select is needed.
 #pragma unroll
         for (int s = 2; s < 64; s <<= 1) {
             const double ib = drcp1(b);
             const double u = a * ib, v = c * ib, w = d * ib;
-            double ub = __shfl_up(u, s, 64), vb = __shfl_up(v, s, 64), wb = __shfl_up(w, s, 64);
-            double uf = __shfl_down(u, s, 64), vf = __shfl_down(v, s, 64), wf = __shfl_down(w, s, 64);
-            if (ln < s) { ub = 0.0; vb = 0.0; wb = 0.0; }
-            if (ln + s >= 64) { uf = 0.0; vf = 0.0; wf = 0.0; }
+            const int au = ((ln - s) & 63) << 2, ad = ((ln + s) & 63) << 2;
+            const double ub = bperm_f64(au, u), vb = bperm_f64(au, v), wb = bperm_f64(au, w);
+            const double uf = bperm_f64(ad, u), vf = bperm_f64(ad, v), wf = bperm_f64(ad, w);
             const double nb = b - a * vb - c * uf;
             const double nd = d - a * wb - c * wf;
             a = -a * ub;
@@ -890,7 +894,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             }
             const float Jf = (float)Jk, hrf = (float)h_r, hzf = (float)h_z;
             // the floor would hide a NaN in J (NaN > Jeps is false): test the solve's output
-            bad |= !(__builtin_isfinite((float)jn) && __builtin_isfinite(hrf) && __builtin_isfinite(hzf));
+            // (a NaN or an infinity in any of the three makes the sum non-finite; the state is
+            // O(1) in units of J0, far from FLT_MAX, so a finite sum cannot overflow)
+            bad |= !__builtin_isfinite(((float)jn + hrf) + hzf);
             const size_t idx = base + k;
             auto& cw = cloud_now();
             cw.sJ[parity ^ 1][idx] = Jf;
@@ -907,6 +913,34 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
                          Azb * fmax(-(double)hzf, 0.0));
         }
         if (edge) kadd(S_ESC, wave_sum64(escl) * cv.J0);
+    }
+
+    // ---- Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660) of the
+    //      segment's cells, one per lane, summed over the lanes in a fixed order ----
+    if (row_valid) {
+        auto& cv = cloud_now();
+        double mf = 0.0, ls = 0.0, net = 0.0, out = 0.0;
+        const int i = i0 + lane;
+        if (i < sg.i1) {
+            const size_t cell = (size_t)j * nr + i;
+            const CellRec* cr = cv.srec + cell;
+            const double fr = seg_sr * cv.mf_scale, fz = seg_sz * cv.mf_scale;
+            cv.sfr[cell] = fr;
+            cv.sfz[cell] = fz;
+            const double dV = cr->dV;
+            mf = dV * sqrt(fr * fr + fz * fz);
+            ls = kFourPi * dV * cr->js;
+            const double ddr = cr->rcom, ddz = cr->zcom - cv.z_center;
+            const double d2 = ddr * ddr + ddz * ddz;
+            double frad = 0.0;
+            if (d2 > 0.0) frad = (fr * ddr + fz * ddz) * drcp1(sqrt(d2));
+            net = dV * frad;
+            out = dV * (frad > 0.0 ? frad : 0.0);
+        }
+        kadd(S_MF, wave_sum64(mf));
+        kadd(S_L, wave_sum64(ls));
+        kadd(S_NET, wave_sum64(net));
+        kadd(S_OUT, wave_sum64(out));
     }
 
     // ---- block partial row: per-wavefront values in LDS, summed in wavefront order ----
@@ -2041,7 +2075,7 @@ int m1lya_batch_set_tiles(m1lya_batch* b, int seg_len, int waves_per_block)
 {
     return bguard(b, [&] {
         if (!b) bfail(nullptr, M1LYA_ERR_ARG, "batch is NULL");
-        if (seg_len < 1 || seg_len > 4096) bfail(b, M1LYA_ERR_ARG, "seg_len out of range [1, 4096]");
+        if (seg_len < 1 || seg_len > 64) bfail(b, M1LYA_ERR_ARG, "seg_len out of range [1, 64]");
         if (waves_per_block < 1 || waves_per_block > kMaxWaves) bfail(b, M1LYA_ERR_ARG, "waves_per_block must be 1..8");
         bchk(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         b->seg_len = seg_len;

@@ -8,7 +8,8 @@
 //  - limited slopes are formed in units of the half-cell width (minmod(s·x, s·y) =
 //    s·minmod(x, y) for s > 0), so they never sit near the bottom of FP32's range
 //    the way per-cm slopes (~1e-22) and their products do;
-//  - the limiter tests signs, not the sign of a product that can underflow;
+//  - the limiter is the median of (A, B, 0), not a test of the sign of a product that can
+//    underflow;
 //  - flux magnitudes are formed from H/J ratios, so |H|² never underflows;
 //  - the thick/thin blend uses 1 - ϵ = τ²/(num + τ²) and (1 - ϵ)/τ = τ/(num + τ²)
 //    (no cancellation, no division by τ);
@@ -42,14 +43,10 @@ __device__ __forceinline__ float fmaxf_(float a, float b) { return __builtin_amd
 __device__ __forceinline__ float fminf_(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -__builtin_inff()); }
 __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
 
-// MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign
-// (sign bits compared, so no product can underflow; a zero argument gives 0).
-__device__ __forceinline__ float minmod(float A, float B)
-{
-    const bool same = (__float_as_int(A) ^ __float_as_int(B)) >= 0;
-    const float m = (fabsf(A) < fabsf(B)) ? A : B;
-    return same ? m : 0.0f;
-}
+// MinMod (src/ConstantsAndFunctions.jl:1571-1582): 0 unless A and B share a sign, else the
+// one of smaller magnitude, which is the median of (A, B, 0): one v_med3_f32, no product
+// that could underflow, and 0 when either argument is 0.
+__device__ __forceinline__ float minmod(float A, float B) { return __builtin_amdgcn_fmed3f(A, B, 0.0f); }
 
 // a = (1-χ)/2 and b = (3χ-1)/2 of the Levermore closure (src/M1Closure.jl:28-33) from
 // f²: χ = (5 - 2s)/3 with s = √(4 - 3f²).
