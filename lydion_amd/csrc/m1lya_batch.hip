@@ -200,13 +200,40 @@ __device__ __forceinline__ double drcp(double x)
     return fma(r, e, r);
 }
 
+// exp(x) for |x| <= kPmax = 40, the range of the clamped exponents of the frequency factors:
+// Cody-Waite reduction x = n ln2 + r with ln2 in two parts (n·ln2_hi is exact for |n| < 2^11),
+// |r| <= ln2/2, the degree-12 Taylor polynomial of exp(r) in Horner form (truncation below
+// 2e-16 relative) and the scale 2^n; no overflow or underflow handling is needed in this range.
+__device__ __forceinline__ double exp_clamped(double x)
+{
+    constexpr double kLog2e = 1.44269504088896338700e+00;
+    constexpr double kLn2Hi = 6.93147180369123816490e-01;
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;
+    const double n = __builtin_rint(x * kLog2e);
+    const double r = fma(-n, kLn2Lo, fma(-n, kLn2Hi, x));
+    double p = 1.0 / 479001600.0;
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)n);
+}
+
 // OneMinusϕ_L_and_χ_L / OneMinusϕ_R_and_χ_R (src/FreqFactors.jl:54-133) in FP64
 __device__ __forceinline__ void omphi_L64(double P, double& o, double& chi)
 {
     if (P >= kPmax) { o = exp(-kPmax); chi = (1.0 - o) * (1.0 / kPmax); return; }
     if (P <= -kPmax) { o = exp(kPmax); chi = (1.0 - o) * (-1.0 / kPmax); return; }
     if (fabs(P) < kPmin) { o = 1.0 - P + 0.5 * (P * P); chi = 1.0 - 0.5 * P; return; }
-    o = exp(-P);
+    o = exp_clamped(-P);
     chi = (1.0 - o) * drcp1(P);
 }
 __device__ __forceinline__ void omphi_R64(double P, double& o, double& chi)
@@ -214,7 +241,7 @@ __device__ __forceinline__ void omphi_R64(double P, double& o, double& chi)
     if (P >= kPmax) { o = exp(kPmax); chi = (1.0 - o) * (1.0 / kPmax); return; }
     if (P <= -kPmax) { o = exp(-kPmax); chi = (1.0 - o) * (-1.0 / kPmax); return; }
     if (fabs(P) < kPmin) { o = 1.0 + P + 0.5 * (P * P); chi = -1.0 - 0.5 * P; return; }
-    o = exp(P);
+    o = exp_clamped(P);
     chi = (1.0 - o) * drcp1(P);
 }
 
@@ -724,7 +751,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
                     oL = 1.0 - P + 0.5 * (P * P);
                     xL = 1.0 - 0.5 * P;
                 } else {
-                    oL = exp(-P);
+                    oL = exp_clamped(-P);
                     xL = (1.0 - oL) * drcp0(P);
                 }
                 if (fabs(Pp) < kPmin) {
@@ -733,7 +760,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
                 } else {
                     // exp(P') = exp(P)·exp(P' - P); P' - P is the rounding of the bin centre
                     const double d = Pp - P;
-                    oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp(Pp);
+                    oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp_clamped(Pp);
                     xRp = (1.0 - oRp) * drcp0(Pp);
                 }
                 Aq[q] = xL * qR;
@@ -1051,23 +1078,27 @@ __global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds
     }
 }
 
-// Step scalars, one block per cloud: the slices are summed per column, then
-// Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673),
-// Compute_f_esc_Lyα_Cylindrical (:969-993) and FindActiveBins
-// (src/UpdateLyaJandH.jl:47-97) for the next step.
+// Step scalars, one block of 256 threads per cloud: the kSlices rows of k_reduce are summed
+// per column in slice order (compensated), then Update_f_Lyα_MF_L_Lyα! normalisation
+// (src/ConstantsAndFunctions.jl:663-673), Compute_f_esc_Lyα_Cylindrical (:969-993) and
+// FindActiveBins (src/UpdateLyaJandH.jl:47-97) for the next step.
 __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds, int n, const double* __restrict__ red,
                                                int* __restrict__ krange, double* __restrict__ scalars,
                                                Ctrl* __restrict__ ctrl, int active_only)
 {
     __shared__ double col[kPartStride];
     __shared__ double tree[256];
-    __shared__ int kmn_s, kmx_s;
+    __shared__ double vout[8];
+    __shared__ int kmn_s, kmx_s, sstep;
     if (!ctrl->live) return;
     const int c = blockIdx.x;
     const Cloud& cl = clouds[c];
     const int nn = cl.nn, ncol = 2 * nn + kNScal, t = threadIdx.x;
+    // the serial tail reads the cursor: load it while the columns are summed
+    if (t == 0) sstep = ctrl->step;
     for (int m = t; m < ncol; m += blockDim.x) {
         KSum acc{0, 0};
+#pragma unroll
         for (int z = 0; z < kSlices; ++z) acc.add(red[((size_t)c * kSlices + z) * kPartStride + m]);
         col[m] = acc.value();
     }
@@ -1085,15 +1116,13 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
         atomicMin(&kmn_s, t);
         atomicMax(&kmx_s, t);
     }
-    __syncthreads();
-    if (t != 0) return;
-    const int used0 = krange[2 * c], used1 = krange[2 * c + 1];
-    if (!active_only) {
+    const int step = sstep;
+    if (!active_only && t == 0) {
         double MF = col[2 * nn], L = col[2 * nn + 1], nt = col[2 * nn + 2], ow = col[2 * nn + 3];
         const double es = col[2 * nn + 4];
         if (L > 0.0) {
-            const double s = 1.0 / (L / kC);
-            MF *= s; nt *= s; ow *= s;
+            const double sc = 1.0 / (L / kC);
+            MF *= sc; nt *= sc; ow *= sc;
         } else {
             MF = 0.0; nt = 0.0; ow = 0.0;
         }
@@ -1105,11 +1134,8 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
         }
         // ΔJ/J = Σ Δν ΔV |J_new - J_start| / E_tot (E_tot: the energy FindActiveBins measures)
         const double rel = (Etot > 0.0) ? col[2 * nn + 5] / Etot : 0.0;
-        const double v[8] = {MF, nt, ow, Ltot, fesc, (double)used0, (double)used1, rel};
-        for (int m = 0; m < 8; ++m) scalars[(size_t)c * 8 + m] = v[m];
-        const int step = ctrl->step;
-        if (ctrl->hist && step < ctrl->cap)
-            for (int m = 0; m < 8; ++m) ctrl->hist[((size_t)step * n + c) * 8 + m] = v[m];
+        vout[0] = MF; vout[1] = nt; vout[2] = ow; vout[3] = Ltot; vout[4] = fesc;
+        vout[5] = (double)krange[2 * c]; vout[6] = (double)krange[2 * c + 1]; vout[7] = rel;
         if (col[2 * nn + 6] != 0.0 && atomicCAS(&ctrl->bad, 0, c + 1) == 0) ctrl->bad_step = step;
         if (ctrl->tol > 0.0 && rel < ctrl->tol && ctrl->steady[c] == 0) {
             ctrl->steady[c] = step + 1;
@@ -1117,6 +1143,13 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
         }
         if (c == 0) ctrl->next = step + 1;
     }
+    __syncthreads();
+    if (!active_only && t < 8) {
+        // the scalars and the history row, one value per thread
+        scalars[(size_t)c * 8 + t] = vout[t];
+        if (ctrl->hist && step < ctrl->cap) ctrl->hist[((size_t)step * n + c) * 8 + t] = vout[t];
+    }
+    if (t != 0) return;
     krange[2 * c] = min(kmn_s, cl.kcore_min);
     krange[2 * c + 1] = max(kmx_s, cl.kcore_max);
 }
