@@ -510,7 +510,7 @@ int m1lya_create(int n_nu, int n_r, int n_z, int precision, m1lya_ctx** out)
         c->ivr = dalloc(c, (size_t)(n_r + 1) * 5); c->ivz = dalloc(c, (size_t)(n_z + 1) * 5);
         c->nu = dalloc(c, n_nu + 1); c->nuC = dalloc(c, n_nu); c->dnu = dalloc(c, n_nu);
         c->inv_dnu = dalloc(c, n_nu); c->inv_nuC = dalloc(c, n_nu);
-        const int W = column_width(n_nu);
+        const int W = column_width(n_nu) * column_waves(n_nu);   // columns per block
         const int ngroups = (int)((nc + W - 1) / W);
         c->col_blocks = column_blocks(n_nu, ngroups);
         c->act_blocks = (int)std::min<size_t>(nc, 1024);

@@ -239,9 +239,12 @@ void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, in
 void launch_advect(hipStream_t s, const AdvectArgs& a);
 // Cell columns per wavefront of the column kernel for an N_ν-bin grid.
 int column_width(int nn);
+// Wavefronts per block of the column kernel for an N_ν-bin grid (the block shares its dgtsv phase).
+int column_waves(int nn);
 // Blocks of the column kernel for ngroups column groups: a fixed count (at most ngroups),
 // so that the partition of the fused reductions, and with it their bits, does not depend
-// on the device.  2048 is one launch wave on MI355X (8 blocks per CU, 256 CUs).
+// on the device.  kColumnBlocks / column_waves(N_ν) blocks are one launch wave on MI355X
+// (8 wavefronts per CU, 256 CUs).
 constexpr int kColumnBlocks = 2048;
 int column_blocks(int nn, int ngroups);
 void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks);

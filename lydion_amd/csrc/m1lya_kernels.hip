@@ -336,23 +336,27 @@ __device__ int gtsv_col(int n, double* __restrict__ dl, double* __restrict__ d, 
 }
 
 // ------------------------------------------------------------------------
-// Column kernel: one wavefront owns W cell columns at a time.
+// Column kernel: a block of NW wavefronts owns W·NW cell columns at a time, W per wavefront.
 //   P1 (lanes over k): emission/absorption (src/ConstantsAndFunctions.jl:483-573),
 //       frequency factors (src/FreqFactors.jl:142-278), tridiagonal rows to LDS
 //       (src/UpdateLyaJandH.jl:178-227);
-//   P2 (lane per column): dgtsv on the LDS column;
+//   P2 (lane per column): dgtsv on the LDS columns, all W·NW of them by the block's first
+//       wavefront: the elimination is a serial chain per column, so the issue cost of P2 is
+//       one wavefront instruction per row step for as many columns as lanes take part;
 //   P3 (lanes over k): J floor, H rescale and causality, old copies
 //       (src/UpdateLyaJandH.jl:335-379), plus the fused step's reductions.
-// LDS per block: 4 arrays x (N_ν-2) rows x W columns (column index fastest,
-// so the lane-per-column sweep is bank-conflict free).
+// LDS per block: 4 arrays x (N_ν-2) rows x W·NW columns (column index fastest, odd row
+// stride, so the lane-per-column sweep is bank-conflict free).
 // ------------------------------------------------------------------------
-template <int W, int KPL>
-__global__ __launch_bounds__(64) void k_column(ColumnArgs a)
+template <int W, int KPL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
 {
     extern __shared__ double lds[];
     const int nn = a.nn, nr = a.nr, nz = a.nz;
     const int n = nn - 2;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int WB = W * NW;   // columns of the block
     const int flags = a.flags;
     const bool EMIT = flags & COL_EMIT, FREQ = flags & COL_FREQ, STOREF = flags & COL_STORE_F;
     const bool SOLVE = flags & COL_SOLVE, HCORR = flags & COL_HCORR, FUSED = flags & COL_FUSED;
@@ -362,11 +366,11 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
     // writes (consecutive rows) and the lane-per-column sweep (stride S, S odd).
     const int S = col_stride(nn);
     double* sDL = lds;
-    double* sD = lds + (size_t)S * W;
-    double* sDU = lds + (size_t)2 * S * W;
-    double* sB = lds + (size_t)3 * S * W;
+    double* sD = lds + (size_t)S * WB;
+    double* sDU = lds + (size_t)2 * S * WB;
+    double* sB = lds + (size_t)3 * S * WB;
     const int ncells = nr * nz;
-    const int ngroups = (ncells + W - 1) / W;
+    const int ngroups = (ncells + WB - 1) / WB;
     const double ct = a.c_tilde, dt = a.dt, dw = a.dw;
     const double inv_ct = 1.0 / ct;
     const double fpc_ict = (kFourPi / kC) * inv_ct;
@@ -384,7 +388,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
         // ================= P1 =================
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const int c = grp * W + w;
+            const int lc = wave * W + w;   // the column's index in the block's LDS
+            const int c = grp * WB + lc;
             if (c >= ncells) continue;
             const size_t base = (size_t)c * nn;
             double sr = 0.0, sz = 0.0;
@@ -507,10 +512,10 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                         // rows of FrequencyUpdateJLyα! (src/UpdateLyaJandH.jl:200-215) in gtsv form:
                         // dl <- A[2:end], du <- C[1:end-1] (src/TridiagonalSolver.jl:45-46)
                         const int row = k - 1;
-                        sD[(size_t)w * S + row] = -(Fk + dw);
-                        sB[(size_t)w * S + row] = -dw * Jo[w][kk];
-                        if (row >= 1) sDL[(size_t)w * S + row - 1] = Fkm;
-                        if (row <= n - 2) sDU[(size_t)w * S + row] = Fkp;
+                        sD[(size_t)lc * S + row] = -(Fk + dw);
+                        sB[(size_t)lc * S + row] = -dw * Jo[w][kk];
+                        if (row >= 1) sDL[(size_t)lc * S + row - 1] = Fkm;
+                        if (row <= n - 2) sDU[(size_t)lc * S + row] = Fkp;
                     }
                 }
             }
@@ -518,8 +523,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
         // ================= P2 =================
         if (SOLVE) {
             __syncthreads();
-            const int c = grp * W + lane;
-            if (lane < W && c < ncells && n > 0) {
+            const int c = grp * WB + lane;
+            if (wave == 0 && lane < WB && c < ncells && n > 0) {
                 int info = gtsv_col(n, sDL + (size_t)lane * S, sD + (size_t)lane * S, sDU + (size_t)lane * S,
                                     sB + (size_t)lane * S);
                 if (info != 0) atomicMax(a.err_flag, 1);
@@ -530,7 +535,8 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
         if (!(EMIT || SOLVE || HCORR)) continue;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const int c = grp * W + w;
+            const int lc = wave * W + w;
+            const int c = grp * WB + lc;
             if (c >= ncells) continue;
             const size_t base = (size_t)c * nn;
             const int i = c % nr, j = c / nr;
@@ -544,7 +550,7 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
                 double jn = Jo[w][kk];
                 double hr = Hr[w][kk], hz = Hz[w][kk];
                 if (SOLVE) {
-                    if (k >= 1 && k <= nn - 2) jn = sB[(size_t)w * S + (k - 1)];
+                    if (k >= 1 && k <= nn - 2) jn = sB[(size_t)lc * S + (k - 1)];
                     else if (!FUSED) jn = a.J[idx];
                 }
                 // the floor below would hide a NaN in J (max(NaN, Jeps) = Jeps): test first
@@ -581,22 +587,33 @@ __global__ __launch_bounds__(64) void k_column(ColumnArgs a)
         }
     }
     if (FUSED) {
-        double* prow = a.part + (size_t)blockIdx.x * a.pstride;
+        // the block's partial row: each wavefront's values into LDS, summed in wavefront order
+        const int ncol = 2 * nn + kStepSums;
+        __syncthreads();
+        double* wrow = lds + (size_t)wave * ncol;
 #pragma unroll
         for (int kk = 0; kk < KPL; ++kk) {
             const int k = lane + 64 * kk;
-            if (k < nn) { prow[k] = e1[kk]; prow[nn + k] = e10[kk]; }
+            if (k < nn) { wrow[k] = e1[kk]; wrow[nn + k] = e10[kk]; }
         }
         conv = wave_sum(conv);
         const double nbad = wave_sum((double)bad);
         if (lane == 0) {
-            prow[2 * nn + SUM_MF] = acc_mf;
-            prow[2 * nn + SUM_L] = acc_L;
-            prow[2 * nn + SUM_NET] = acc_net;
-            prow[2 * nn + SUM_OUT] = acc_out;
-            prow[2 * nn + SUM_ESC] = acc_esc;
-            prow[2 * nn + SUM_CONV] = conv;
-            prow[2 * nn + SUM_BAD] = nbad;
+            wrow[2 * nn + SUM_MF] = acc_mf;
+            wrow[2 * nn + SUM_L] = acc_L;
+            wrow[2 * nn + SUM_NET] = acc_net;
+            wrow[2 * nn + SUM_OUT] = acc_out;
+            wrow[2 * nn + SUM_ESC] = acc_esc;
+            wrow[2 * nn + SUM_CONV] = conv;
+            wrow[2 * nn + SUM_BAD] = nbad;
+        }
+        __syncthreads();
+        double* prow = a.part + (size_t)blockIdx.x * a.pstride;
+        for (int t = threadIdx.x; t < ncol; t += 64 * NW) {
+            double v = lds[t];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) v += lds[(size_t)w * ncol + t];
+            prow[t] = v;
         }
     }
 }
@@ -812,11 +829,13 @@ void launch_advect(hipStream_t s, const AdvectArgs& a)
     hipLaunchKernelGGL((k_advect<BI, BJ>), grid, dim3(64, BI, BJ), 0, s, a);
 }
 
-template <int W, int KPL>
+template <int W, int KPL, int NW>
 static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
 {
-    size_t lds = 4 * (size_t)col_stride(a.nn) * W * sizeof(double);
-    hipLaunchKernelGGL((k_column<W, KPL>), dim3(nblocks), dim3(64), lds, s, a);
+    const size_t rows = 4 * (size_t)col_stride(a.nn) * W * NW;
+    const size_t sums = (size_t)NW * (2 * a.nn + kStepSums);
+    const size_t lds = std::max(rows, sums) * sizeof(double);
+    hipLaunchKernelGGL((k_column<W, KPL, NW>), dim3(nblocks), dim3(64 * NW), lds, s, a);
 }
 
 // Up to 256 bins (four per lane) four cell columns share a wavefront; beyond, up to 576 bins
@@ -824,29 +843,34 @@ static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
 // per-lane registers and the LDS rows within bounds.
 int column_width(int nn) { return nn <= 256 ? kColumnW : 1; }
 
+// Wavefronts per block of the column kernel: the block's W·NW columns fill 64 KB of LDS up to
+// 128 bins (16 columns), 64 KB up to 256 bins (8 columns); wider grids keep one wavefront
+// (one column of up to 18 KB).  Two blocks per CU then hold 8 wavefronts.
+int column_waves(int nn) { return nn <= 128 ? 4 : (nn <= 256 ? 2 : 1); }
+
 // A grid of exactly one wave of resident blocks: every block loops over the same number of
 // column groups and they all start together.  A larger grid left a partial second wave of
 // blocks starting late (2560 blocks against 2048 resident ones on the 256x256x128 cone
-// cloud: 1.15 ms against 0.81 ms for the kernel).
+// cloud: 1.15 ms against 0.81 ms for the kernel).  The count is fixed per block shape, so the
+// partition of the fused reductions does not depend on the device.
 int column_blocks(int nn, int ngroups)
 {
-    (void)nn;
-    return std::max(1, std::min(ngroups, kColumnBlocks));
+    return std::max(1, std::min(ngroups, kColumnBlocks / column_waves(nn)));
 }
 
 void launch_column(hipStream_t s, const ColumnArgs& a, int nblocks)
 {
     const int kpl = (a.nn + 63) / 64;
     switch (kpl) {
-    case 1: launch_column_t<kColumnW, 1>(s, a, nblocks); break;
-    case 2: launch_column_t<kColumnW, 2>(s, a, nblocks); break;
-    case 3: launch_column_t<kColumnW, 3>(s, a, nblocks); break;
-    case 4: launch_column_t<kColumnW, 4>(s, a, nblocks); break;
-    case 5: launch_column_t<1, 5>(s, a, nblocks); break;
-    case 6: launch_column_t<1, 6>(s, a, nblocks); break;
-    case 7: launch_column_t<1, 7>(s, a, nblocks); break;
-    case 8: launch_column_t<1, 8>(s, a, nblocks); break;
-    default: launch_column_t<1, 9>(s, a, nblocks); break;
+    case 1: launch_column_t<kColumnW, 1, 4>(s, a, nblocks); break;
+    case 2: launch_column_t<kColumnW, 2, 4>(s, a, nblocks); break;
+    case 3: launch_column_t<kColumnW, 3, 2>(s, a, nblocks); break;
+    case 4: launch_column_t<kColumnW, 4, 2>(s, a, nblocks); break;
+    case 5: launch_column_t<1, 5, 1>(s, a, nblocks); break;
+    case 6: launch_column_t<1, 6, 1>(s, a, nblocks); break;
+    case 7: launch_column_t<1, 7, 1>(s, a, nblocks); break;
+    case 8: launch_column_t<1, 8, 1>(s, a, nblocks); break;
+    default: launch_column_t<1, 9, 1>(s, a, nblocks); break;
     }
 }
 
