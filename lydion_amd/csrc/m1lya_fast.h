@@ -131,21 +131,17 @@ __device__ __forceinline__ Side32 side_state(float fr, float fz)
 {
     Side32 s;
     const float f2 = fr * fr + fz * fz;
-    if (f2 > kFmin32 * kFmin32) {
-        const float i = frsq(f2);
-        if (f2 > 1.0f) {
-            fr *= i; fz *= i;
-            s.f = 1.0f; s.nr = fr; s.nz = fz;
-        } else {
-            s.f = f2 * i; s.nr = fr * i; s.nz = fz * i;
-        }
-        const float fc = fminf_(s.f, kFmax32);
-        levermore_ab(fc * fc, s.a, s.b);
-    } else {
-        s.f = 0.0f; s.nr = 0.0f; s.nz = 0.0f;
-        s.a = kThird; s.b = 0.0f;
-    }
-    s.fr = fr; s.fz = fz;
+    // No branches: below |f| = 1e-10 the closure is isotropic without a special case, since
+    // 4 - 3 f² then rounds to 4, so b = 0 and a = 1/3 exactly and the (finite) direction
+    // drops out of D; above |f| = 1 the limited f is the direction itself.
+    const float i = frsq(fmaxf_(f2, kFmin32 * kFmin32));
+    s.nr = fr * i;
+    s.nz = fz * i;
+    const bool lim = f2 > 1.0f;
+    s.fr = lim ? s.nr : fr;
+    s.fz = lim ? s.nz : fz;
+    s.f = fminf_(f2 * i, 1.0f);
+    levermore_ab(fminf_(f2, kFmax32 * kFmax32), s.a, s.b);
     return s;
 }
 
@@ -240,11 +236,16 @@ __device__ inline Face32 interface_flux(int dir, int hll, const Stencil32& s, co
         FHr = 0.5f * (FHr_L + FHr_R) - he * (Hr_R - Hr_L);
         FHz = 0.5f * (FHz_L + FHz_R) - he * (Hz_R - Hz_L);
     }
-    const bool up = (u_over_c < 0.0f);
     Face32 r;
-    r.FJ = FJ + u_over_c * (up ? J_R : J_L);
-    r.FHr = FHr + u_over_c * (up ? Hr_R : Hr_L);
-    r.FHz = FHz + u_over_c * (up ? Hz_R : Hz_L);
+    r.FJ = FJ;
+    r.FHr = FHr;
+    r.FHz = FHz;
+    if (u_over_c != 0.0f) {   // the face velocity is one value for all bins: a uniform branch
+        const bool up = (u_over_c < 0.0f);
+        r.FJ += u_over_c * (up ? J_R : J_L);
+        r.FHr += u_over_c * (up ? Hr_R : Hr_L);
+        r.FHz += u_over_c * (up ? Hz_R : Hz_L);
+    }
     r.J_L = J_L; r.J_R = J_R; r.Hz_R = Hz_R;
     // clamp(1 - D_rr - D_ZZ, 0, 1)·J with 1 - D_rr - D_ZZ = D_ϕϕ = a
     r.kphi_L = fclamp(SL.a, 0.0f, 1.0f) * J_L;
