@@ -234,15 +234,21 @@ def load_valu(kernel: str, workload: str):
 def roofline(stats, dom, upd_per_step, steps, workload):
     """The dominant kernel against the HBM roofline.  Beside it, the share of its duration that
     issuing its VALU instructions takes (one wave64 VALU instruction per CU per cycle): where
-    that share exceeds the HBM fraction, instruction issue, not bandwidth, limits the kernel."""
-    launches, total_ms = stats[dom]
-    avg_s = total_ms * 1e-3 / launches
+    that share exceeds the HBM fraction, instruction issue, not bandwidth, limits the kernel.
+    For the batched step, k_step and k_step_pair (clouds of N_ν <= 32, launched before k_step in
+    the same step) are taken together: their summed time per step against all the step's units."""
+    parts = [k for k in ([dom, "k_step_pair"] if dom == "k_step" else [dom]) if k in stats]
+    total_ms = sum(stats[k][1] for k in parts)
+    avg_s = total_ms * 1e-3 / steps
     achieved = ALGO_BYTES[dom] * upd_per_step / avg_s / 1e9
-    valu = load_valu(dom, workload)
+    valus = [load_valu(k, workload) for k in parts]
+    valu = None if any(v is None for v in valus) else sum(valus)
     issue = None if valu is None else valu / (CUS * CLOCK_HZ) / avg_s
     frac = achieved / HBM_PEAK_GBS
-    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(frac, 4), "traffic": load_traffic(dom, workload),
+    traffics = [load_traffic(k, workload) for k in parts]
+    traffic = None if any(t is None for t in traffics) else sum(traffics)
+    return {"bound": "hbm", "kernel": "+".join(parts), "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * upd_per_step,
             "avg_launch_ms": round(1e3 * avg_s, 4),
             "valu_issue_frac": None if issue is None else round(issue, 4),

@@ -79,6 +79,7 @@ struct BinRec {
 
 struct Cloud {
     int nn, nr, nz, R;
+    int pair;                      // N_ν <= 32: two cells per wavefront (k_step<..., 1, true>)
     long long o3, oc, ofr, ofz, obin, oivr, oivz;
     int blk0, blk1;                // this cloud's partial rows (= its blocks)
     int kcore_min, kcore_max, recoil, hll;
@@ -155,6 +156,11 @@ __device__ __forceinline__ double dpp_f64(double v)
 __device__ __forceinline__ double from_prev_lane(double v) { return dpp_f64<0x138>(v); }   // 0 in lane 0
 __device__ __forceinline__ double from_next_lane(double v) { return dpp_f64<0x130>(v); }   // 0 in lane 63
 
+__device__ __forceinline__ float bperm_f32(int addr, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
 // The value of lane addr/4 (ds_bpermute, any lane of the wavefront).
 __device__ __forceinline__ double bperm_f64(int addr, double v)
 {
@@ -178,6 +184,18 @@ __device__ __forceinline__ double wave_sum64(double v)
     v += dpp_f64<0x114>(v);
     v += dpp_f64<0x118>(v);
     return (lane_value(v, 15) + lane_value(v, 31)) + (lane_value(v, 47) + lane_value(v, 63));
+}
+
+// Sum over each half-wavefront (lanes 0-31, 32-63), in every lane of the half; the same
+// fixed order as wave_sum64 within a half.
+__device__ __forceinline__ double half_sum32(double v, int half)
+{
+    v += dpp_f64<0x111>(v);
+    v += dpp_f64<0x112>(v);
+    v += dpp_f64<0x114>(v);
+    v += dpp_f64<0x118>(v);
+    const double s0 = lane_value(v, 15) + lane_value(v, 31), s1 = lane_value(v, 47) + lane_value(v, 63);
+    return half ? s1 : s0;
 }
 
 // 1/x from the hardware estimate (relative error ~2^-24): drcp0 as is, for values that
@@ -395,12 +413,17 @@ constexpr size_t step_lds_bytes(int wpb, int nq)
            sizeof(float) * 3 * 64 * nq * (size_t)(2 * wpb + 5);
 }
 
-template <bool kShear, int NQ>
-__global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const Cloud* __restrict__ clouds, Arrays A,
-                                                            const Seg* __restrict__ segs, int nblk, int wpb,
-                                                            int parity, const int* __restrict__ krange,
+// kPair (with NQ = 1, clouds of N_ν <= 32): the two halves of a wavefront hold two neighbouring
+// cells of its row, i (lanes 0-31) and i+1 (lanes 32-63), one bin per lane, so that such clouds
+// fill every lane; the march advances two cells at a time.  The launch covers the blocks
+// [blk0, blk1) of the batch.
+template <bool kShear, int NQ, bool kPair>
+__global__ __launch_bounds__(64 * kMaxWaves, NQ <= 2 ? 4 : 2) void k_step(const Cloud* __restrict__ clouds, Arrays A,
+                                                            const Seg* __restrict__ segs, int blk0, int blk1,
+                                                            int wpb, int parity, const int* __restrict__ krange,
                                                             double* __restrict__ part, const Ctrl* __restrict__ ctrl)
 {
+    static_assert(!kPair || NQ == 1, "pair mode holds one bin per lane");
     if (ctrl->halt) return;
     constexpr int kB = 64 * NQ;          // bins a wavefront holds
     constexpr int kSh = 2 * kB + 8;      // LDS row of the per-wavefront sums
@@ -410,13 +433,15 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
     double (*sacc)[2 * kNScal] = reinterpret_cast<double (*)[2 * kNScal]>(&sh[wpb][0]);
     float (*zrow)[3][kB] = reinterpret_cast<float (*)[3][kB]>(&sacc[wpb][0]);
     float (*zflx)[3][kB] = zrow + (wpb + 4);
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    if (lb >= nblk) return;
+    const int lb = blk0 + xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= blk1) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const Seg sg = segs[(size_t)lb * wpb + wave];
     const int cidx = segs[(size_t)lb * wpb].cloud;
     const int j0 = segs[(size_t)lb * wpb].j;
+    constexpr int kStep = kPair ? 2 : 1;            // cells per step of the march
+    const int half = kPair ? (lane >> 5) : 0;       // the lane's cell is i + half
 
     // per-wavefront compensated sums of M_F, L, radial net/out and f_esc, kept by lane 0
     // in LDS (s, c pairs) rather than in registers of every lane
@@ -430,8 +455,9 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         }
     };
     const Cloud& cl = clouds[cidx];
-    // the wavefront's row of sh accumulates the active-bin energies e1[k], e10[k]
-    for (int t = lane; t < 2 * cl.nn; t += 64) sh[wave][t] = 0.0;
+    // the wavefront's row of sh accumulates the active-bin energies e1[k], e10[k] (in pair mode
+    // each half its own: e1 at [32 half + k], e10 at [64 + 32 half + k], folded at the end)
+    for (int t = lane; t < (kPair ? 128 : 2 * cl.nn); t += 64) sh[wave][t] = 0.0;
     const int nn = cl.nn, nr = cl.nr, nz = cl.nz, R = cl.R, j = sg.j;
     const bool row_valid = (j < nz);                    // rows above the cloud are ghost rows
     const int nvalid = min(wpb, nz - j0);
@@ -454,7 +480,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         LaneBins<NQ> B;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            B.kb[q] = ln * R + q;
+            B.kb[q] = kPair ? (ln & 31) : ln * R + q;
             B.val[q] = (q < R) && (B.kb[q] < nn);
             B.act[q] = B.val[q] && B.kb[q] >= kmin && B.kb[q] <= kmax;
         }
@@ -468,6 +494,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
     // N_Z + 1 are never part of a needed stencil and are clamped.
     auto load_zrow = [&](int i, int row, int slot) {
         row = min(row, nz + 1);
+        if (kPair) i = min(i + half, nr - 1);   // the lane's cell (a cell past the row is not used)
         auto& cp = cloud_now();
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -491,12 +518,14 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         for (int m = 0; m < 4; ++m) {
             wJ[q][m] = 0.f; wR[q][m] = 0.f; wZ[q][m] = 0.f;
             // cells i0-2, i0-1, i0 in slots 1..3: the first shift and load of the
-            // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1)
-            if (row_valid && L.val[q] && m >= 1) {
+            // march (i = i0-1) make the window (i0-2, i0-1, i0, i0+1).  In pair mode the lane's
+            // cells c-1, c (c = i0-2+half) in slots 2, 3: the first shift by two and two loads
+            // make (c-1, c, c+1, c+2).
+            if (row_valid && L.val[q] && m >= (kPair ? 2 : 1)) {
                 auto& cp = cloud_now();
                 float vr, vz;
-                ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, L.kb[q], i0 - 3 + m, bcr, wJ[q][m],
-                     vr, vz);
+                ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, L.kb[q],
+                     kPair ? i0 - 5 + m + half : i0 - 3 + m, bcr, wJ[q][m], vr, vz);
                 m1f::flux_ratios(wJ[q][m], vr, vz, wR[q][m], wZ[q][m]);
             }
         }
@@ -509,15 +538,21 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
     int bad = 0;
     // lane t: Σ_k Δν (ΔH_r, ΔH_Z) of the emission/absorption update of cell i0 + t
     double seg_sr = 0.0, seg_sz = 0.0;
+    // pair mode: flux and K_ϕϕ of the right interface of the other half's cell, one step back
+    float Xp[3] = {0.f, 0.f, 0.f}, Xkp = 0.f;
 
     // All wavefronts of the block march over the same cells [i0, i1) in lockstep;
-    // i = i0-1 only evaluates the left r interface of cell i0.
-    for (int i = i0 - 1; i < sg.i1; ++i) {
+    // i = i0-1 only evaluates the left r interface of cell i0 (pair mode: i = i0-2, whose
+    // upper half evaluates it).
+    for (int i = i0 - kStep; i < sg.i1; i += kStep) {
         int ln = lane;
         asm volatile("" : "+v"(ln));
         L = lane_bins(ln);
         const bool first = (i < i0);
-        const size_t cell = (size_t)j * nr + i;   // used only for valid rows, i >= i0
+        const int ci = i + half;                          // the lane's cell
+        const bool cell_ok = !kPair || ci < sg.i1;       // pair mode: i+1 may lie past the segment
+        const int cc = kPair ? min(max(ci, 0), nr - 1) : i;   // its index for loads (clamped in pair mode)
+        const size_t cell = (size_t)j * nr + cc;   // used only for valid rows, i >= i0
         const size_t base = cell * nn;
 
         float af_c[NQ] = {};   // α_F of cell i (the left side of its right r interface)
@@ -532,31 +567,61 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             kip[q] = 0.f; kimN[q] = 0.f;
             if (!row_valid || !L.val[q]) continue;
             const int k = L.kb[q];
-            wJ[q][0] = wJ[q][1]; wJ[q][1] = wJ[q][2]; wJ[q][2] = wJ[q][3];
-            wR[q][0] = wR[q][1]; wR[q][1] = wR[q][2]; wR[q][2] = wR[q][3];
-            wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
-            {
+            if (kPair) {
+                wJ[q][0] = wJ[q][2]; wJ[q][1] = wJ[q][3];
+                wR[q][0] = wR[q][2]; wR[q][1] = wR[q][3];
+                wZ[q][0] = wZ[q][2]; wZ[q][1] = wZ[q][3];
+#pragma unroll
+                for (int m = 2; m < 4; ++m) {
+                    float vr, vz;
+                    ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, k, ci + m - 1, bcr, wJ[q][m], vr,
+                         vz);
+                    m1f::flux_ratios(wJ[q][m], vr, vz, wR[q][m], wZ[q][m]);
+                }
+            } else {
+                wJ[q][0] = wJ[q][1]; wJ[q][1] = wJ[q][2]; wJ[q][2] = wJ[q][3];
+                wR[q][0] = wR[q][1]; wR[q][1] = wR[q][2]; wR[q][2] = wR[q][3];
+                wZ[q][0] = wZ[q][1]; wZ[q][1] = wZ[q][2]; wZ[q][2] = wZ[q][3];
                 float vr, vz;
                 ld_r(cp.sJ[parity], cp.sHr[parity], cp.sHz[parity], nn, nr, j, k, i + 2, bcr, wJ[q][3], vr, vz);
                 m1f::flux_ratios(wJ[q][3], vr, vz, wR[q][3], wZ[q][3]);
             }
-            af_c[q] = aF[((size_t)j * nr + max(i, 0)) * nn + k];
+            af_c[q] = aF[((size_t)j * nr + max(cc, 0)) * nn + k];
             if (!L.act[q]) continue;
             m1f::Stencil32 st;
             st.J[0] = wJ[q][1]; st.J[1] = wJ[q][2]; st.J[2] = wJ[q][3]; st.J[3] = wJ[q][0];
             st.fr[0] = wR[q][1]; st.fr[1] = wR[q][2]; st.fr[2] = wR[q][3]; st.fr[3] = wR[q][0];
             st.fz[0] = wZ[q][1]; st.fz[1] = wZ[q][2]; st.fz[2] = wZ[q][3]; st.fz[3] = wZ[q][0];
             const float aL = af_c[q];
-            const float aR = aF[((size_t)j * nr + min(i + 1, nr - 1)) * nn + k];
-            const float u = cp.sucr[(size_t)j * (nr + 1) + i + 1];
+            // the right face of the lane's cell (pair mode: clamped to the row's faces for the
+            // lanes whose cell is not used)
+            const int fr_ = kPair ? min(max(ci + 1, 0), nr) : i + 1;
+            const float aR = aF[((size_t)j * nr + min(fr_, nr - 1)) * nn + k];
+            const float u = cp.sucr[(size_t)j * (nr + 1) + fr_];
             const m1f::Face32 f =
-                m1f::interface_flux(0, hll, st, cp.sivr + (size_t)(i + 1) * m1f::kIvStride, aL, aR, u);
+                m1f::interface_flux(0, hll, st, cp.sivr + (size_t)fr_ * m1f::kIvStride, aL, aR, u);
             FR[q][0] = f.FJ; FR[q][1] = f.FHr; FR[q][2] = f.FHz;
             kip[q] = f.kphi_L;
-            kimN[q] = (i + 1 == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
+            kimN[q] = (fr_ == 0) ? m1f::kphi_inner_axis(f.J_R, f.Hz_R) : f.kphi_R;
         }
         }
-        if (first) {
+        if constexpr (kPair) {
+            // the left interface of cell i+1 is the right one of cell i (the other half, this
+            // step); that of cell i is the right one of cell i-1 (the other half, last step)
+            const int sw = (lane ^ 32) << 2;
+            float X[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) X[m] = bperm_f32(sw, FR[0][m]);
+            const float Xk = bperm_f32(sw, kimN[0]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                FL[0][m] = half ? X[m] : Xp[m];
+                Xp[m] = X[m];
+            }
+            kimR[0] = half ? Xk : Xkp;
+            Xkp = Xk;
+            if (first) continue;
+        } else if (first) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
@@ -611,10 +676,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
                     st.fr[m] = zrow[slot][1][p];
                     st.fz[m] = zrow[slot][2][p];
                 }
-                const float aL = aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
-                const float aR = aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+                const float aL = aF[((size_t)max(g - 1, 0) * nr + cc) * nn + k];
+                const float aR = aF[((size_t)min(g, nz - 1) * nr + cc) * nn + k];
                 const m1f::Face32 r = m1f::interface_flux(1, hll, st, cp.sivz + (size_t)g * m1f::kIvStride, aL, aR,
-                                                          cp.sucz[(size_t)g * nr + i]);
+                                                          cp.sucz[(size_t)g * nr + cc]);
                 zflx[f][0][p] = r.FJ;
                 zflx[f][1][p] = r.FHr;
                 zflx[f][2][p] = r.FHz;
@@ -659,8 +724,10 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
                 // EnforceCausalReducedFlux: src/AdvectJandH.jl:478-493
                 m1f::causal_limit(Jn, Hrn, Hzn);
             }
-            FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
-            kimR[q] = kimN[q];
+            if (!kPair) {
+                FL[q][0] = FR[q][0]; FL[q][1] = FR[q][1]; FL[q][2] = FR[q][2];
+                kimR[q] = kimN[q];
+            }
             jo[q] = Jn; hr[q] = Hrn; hz[q] = Hzn;
         }
 
@@ -691,13 +758,28 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             hr[q] -= hr[q] * gH;
             hz[q] -= hz[q] * gH;
         }
-        sr = wave_sum64(sr);
-        sz = wave_sum64(sz);
         // the cell's Σ Δν ΔH goes to lane i - i0: the per-cell terms of Update_f_Lyα_MF_L_Lyα!
         // are formed after the march, one cell per lane, instead of by the whole wavefront
-        if (ln == i - i0) {
-            seg_sr = sr;
-            seg_sz = sz;
+        if constexpr (kPair) {
+            sr = half_sum32(sr, half);
+            sz = half_sum32(sz, half);
+            const double sr0 = lane_value(sr, 0), sz0 = lane_value(sz, 0);
+            const double sr1 = lane_value(sr, 32), sz1 = lane_value(sz, 32);
+            if (ln == i - i0) {
+                seg_sr = sr0;
+                seg_sz = sz0;
+            }
+            if (ln == i + 1 - i0 && i + 1 < sg.i1) {
+                seg_sr = sr1;
+                seg_sz = sz1;
+            }
+        } else {
+            sr = wave_sum64(sr);
+            sz = wave_sum64(sz);
+            if (ln == i - i0) {
+                seg_sr = sr;
+                seg_sz = sz;
+            }
         }
 
         // ================= frequency factors (FP64) =================
@@ -840,7 +922,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             ra[0] = a0; rb[0] = b0; rc[0] = c0; rd[0] = d0;
             ra[1] = a2; rb[1] = b2; rc[1] = c2; rd[1] = d2;
         }
-        if (R >= 2) {
+        if (NQ >= 2 && R >= 2) {
             // eliminate the odd row of each lane's pair into the even rows
             const double ib1 = drcp1(rb[1]);
             ua = ra[1] * ib1; uc = rc[1] * ib1; ud = rd[1] * ib1;
@@ -870,11 +952,14 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         // above 63 - s have c = 0 at level s (bin 0, bin N_ν-1 and the bins past N_ν are
         // identity rows, and level s/2 zeroed the next s/2 lanes by multiplying with them),
         // and the values are finite, so no select is needed.
+        // (pair mode: each half is its own system of 32 rows; the lanes wrap within the half, and
+        // level 1's DPP moves across the halves meet bin 0's a = 0 and bin 31's c = 0)
 #pragma unroll
-        for (int s = 2; s < 64; s <<= 1) {
+        for (int s = 2; s < (kPair ? 32 : 64); s <<= 1) {
             const double ib = drcp1(b);
             const double u = a * ib, v = c * ib, w = d * ib;
-            const int au = ((ln - s) & 63) << 2, ad = ((ln + s) & 63) << 2;
+            const int au = (kPair ? (((ln - s) & 31) | (ln & 32)) : ((ln - s) & 63)) << 2;
+            const int ad = (kPair ? (((ln + s) & 31) | (ln & 32)) : ((ln + s) & 63)) << 2;
             const double ub = bperm_f64(au, u), vb = bperm_f64(au, v), wb = bperm_f64(au, w);
             const double uf = bperm_f64(ad, u), vf = bperm_f64(ad, v), wf = bperm_f64(ad, w);
             const double nb = b - a * vb - c * uf;
@@ -888,7 +973,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         xs[0] = d * drcp1(b);
 #pragma unroll
         for (int q = 1; q < NQ; ++q) xs[q] = 0.0;
-        if (R >= 2) {
+        if (NQ >= 2 && R >= 2) {
             const double x0n = from_next_lane(xs[0]);
             xs[1] = ud - ua * xs[0] - uc * x0n;
             if (NQ == 4 && R == 4) {
@@ -906,7 +991,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         double escl = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            if (!L.val[q]) continue;
+            if (!L.val[q] || !cell_ok) continue;
             const int k = L.kb[q];
             // J floor, H rescale and causality: src/UpdateLyaJandH.jl:335-379
             const double jn = (k >= 1 && k <= nn - 2) ? xs[q] : jo[q];
@@ -931,15 +1016,16 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
             cw.sHz[parity ^ 1][idx] = hzf;
             const double Jd = Jf, dk = bins[k].dnu;
             conv += dk * (double)fabsf(Jf - wJ[q][1]) * cr.dV;
-            sh[wave][k] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
-            sh[wave][nn + k] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
+            const int e1k = kPair ? 32 * half + k : k, e10k = kPair ? 64 + 32 * half + k : nn + k;
+            sh[wave][e1k] += dk * ((Jd - kJeps) > 0.0 ? (Jd - kJeps) : 0.0) * cr.dV;
+            sh[wave][e10k] += dk * ((Jd - 10.0 * kJeps) > 0.0 ? (Jd - 10.0 * kJeps) : 0.0) * cr.dV;
             // Compute_f_esc_Cylindrical terms (src/ConstantsAndFunctions.jl:937-953)
             if (edge)
                 escl += dk * kFourPi *
                         (cr.A_rout * fmax((double)hrf, 0.0) + cr.A_ztop * fmax((double)hzf, 0.0) +
                          Azb * fmax(-(double)hzf, 0.0));
         }
-        if (edge) kadd(S_ESC, wave_sum64(escl) * cv.J0);
+        if (kPair ? (__ballot(edge) != 0) : edge) kadd(S_ESC, wave_sum64(escl) * cv.J0);
     }
 
     // ---- Update_f_Lyα_MF_L_Lyα! per-cell terms (src/ConstantsAndFunctions.jl:612-660) of the
@@ -978,6 +1064,19 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ == 2 ? 4 : 2) void k_step(const 
         const double nb = (double)__popcll(__ballot(bad != 0));
         kadd(S_CONV, cs * cb.J0);
         kadd(S_BAD, nb);
+    }
+    if constexpr (kPair) {
+        // fold the two halves' active-bin energies into the row layout e1[k], e10[k]
+        double e1 = 0.0, e10 = 0.0;
+        if (lane < 32) {
+            e1 = sh[wave][lane] + sh[wave][32 + lane];
+            e10 = sh[wave][64 + lane] + sh[wave][96 + lane];
+        }
+        __syncthreads();
+        if (lane < nnb) {
+            sh[wave][lane] = e1;
+            sh[wave][nnb + lane] = e10;
+        }
     }
     for (int t = lane; t < 2 * nnb; t += 64) sh[wave][t] *= cb.J0;
     if (lane == 0) {
@@ -1223,6 +1322,7 @@ struct m1lya_batch {
     Seg* d_segs = nullptr;
     size_t segs_cap = 0;
     int nblk = 0, grid = 0;
+    int nblk_pair = 0;                     // blocks [0, nblk_pair) belong to clouds in pair mode
     double* part = nullptr;
     size_t part_rows_cap = 0;
     double* red = nullptr;
@@ -1311,12 +1411,18 @@ void check_cloud(m1lya_batch* b, int c)
 }
 
 // Work table of the current tile shape: per cloud, row segments of seg_len cells,
-// wpb of them (same i range, consecutive rows) per block; blocks never mix clouds.
+// wpb of them (same i range, consecutive rows) per block; blocks never mix clouds.  The
+// blocks of clouds in pair mode come first: they are launched on their own.
 void build_tiles(m1lya_batch* b)
 {
     std::vector<Seg> segs;
     int blk = 0;
-    for (int c = 0; c < b->n; ++c) {
+    std::vector<int> order;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < b->n; ++c)
+            if ((b->cl[c].pair != 0) == (pass == 0)) order.push_back(c);
+    int nblk_pair = 0;
+    for (int c : order) {
         Cloud& cl = b->cl[c];
         cl.blk0 = blk;
         const int nseg_r = (cl.nr + b->seg_len - 1) / b->seg_len;
@@ -1329,8 +1435,10 @@ void build_tiles(m1lya_batch* b)
                 ++blk;
             }
         cl.blk1 = blk;
+        if (cl.pair) nblk_pair = blk;
     }
     b->nblk = blk;
+    b->nblk_pair = nblk_pair;
     b->grid = (blk + 7) / 8 * 8;
     if (segs.size() > b->segs_cap) {
         bfree(b, b->d_segs);
@@ -1448,26 +1556,41 @@ void collect_stats(m1lya_batch* b)
 void launch_step_kernels(m1lya_batch* b, int parity)
 {
     hipStream_t s = b->stream;
-    {
+    const dim3 block(64 * b->wpb);
+    if (b->nblk_pair > 0) {
+        // clouds of N_ν <= 32, two cells per wavefront
+        BProf p(b, "k_step_pair", true);
+        const size_t lds = step_lds_bytes(b->wpb, 1);
+        const dim3 grid((b->nblk_pair + 7) / 8 * 8);
+        if (b->shear)
+            hipLaunchKernelGGL((k_step<true, 1, true>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, 0, b->nblk_pair,
+                               b->wpb, parity, b->krange, b->part, b->ctrl);
+        else
+            hipLaunchKernelGGL((k_step<false, 1, true>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, 0,
+                               b->nblk_pair, b->wpb, parity, b->krange, b->part, b->ctrl);
+        p.done();
+    }
+    if (b->nblk > b->nblk_pair) {
         BProf p(b, "k_step", true);
         // the variant with the reference-row path for sheared / anisotropic velocity gradients only
         // when a cell of the batch needs it: it costs registers the common path does not pay for
         // and the four-bins-per-lane variant only when a cloud has more than 128 bins
         const size_t lds = step_lds_bytes(b->wpb, b->wide ? 4 : 2);
-        const dim3 grid(b->grid), block(64 * b->wpb);
+        const int b0 = b->nblk_pair, b1 = b->nblk;
+        const dim3 grid((b1 - b0 + 7) / 8 * 8);
         if (b->wide) {
             if (b->shear)
-                hipLaunchKernelGGL((k_step<true, 4>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
-                                   parity, b->krange, b->part, b->ctrl);
+                hipLaunchKernelGGL((k_step<true, 4, false>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b0, b1,
+                                   b->wpb, parity, b->krange, b->part, b->ctrl);
             else
-                hipLaunchKernelGGL((k_step<false, 4>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk,
+                hipLaunchKernelGGL((k_step<false, 4, false>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b0, b1,
                                    b->wpb, parity, b->krange, b->part, b->ctrl);
         } else if (b->shear) {
-            hipLaunchKernelGGL((k_step<true, 2>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
-                               parity, b->krange, b->part, b->ctrl);
+            hipLaunchKernelGGL((k_step<true, 2, false>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b0, b1,
+                               b->wpb, parity, b->krange, b->part, b->ctrl);
         } else {
-            hipLaunchKernelGGL((k_step<false, 2>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b->nblk, b->wpb,
-                               parity, b->krange, b->part, b->ctrl);
+            hipLaunchKernelGGL((k_step<false, 2, false>), grid, block, lds, s, b->d_cl, b->A, b->d_segs, b0, b1,
+                               b->wpb, parity, b->krange, b->part, b->ctrl);
         }
         p.done();
     }
@@ -1654,6 +1777,7 @@ int m1lya_batch_create(int n_clouds, const int* n_nu, const int* n_r, const int*
             std::memset(&cl, 0, sizeof(Cloud));
             cl.nn = n_nu[c]; cl.nr = n_r[c]; cl.nz = n_z[c];
             cl.R = cl.nn <= 64 ? 1 : cl.nn <= 128 ? 2 : 4;
+            cl.pair = cl.nn <= 32;
             if (cl.R == 4) b->wide = true;
             const size_t ncell = (size_t)cl.nr * cl.nz;
             cl.o3 = (long long)b->n3; b->n3 += ncell * cl.nn;

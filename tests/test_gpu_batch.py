@@ -95,6 +95,34 @@ def test_outflow_bottom_and_reflecting_right(lut):
     _check_history(hb, h64, "outflow-bottom")
 
 
+@pytest.mark.parametrize("hll,bc,n_nu,tiles", [(0, "reflect", 32, (16, 8)), (1, "outflow", 32, (7, 3)),
+                                                (0, "outflow", 24, (5, 2)), (1, "reflect", 18, (1, 4))])
+def test_pair_mode_vs_fp64(lut, hll, bc, n_nu, tiles):
+    """Clouds of N_ν <= 32 run two cells per wavefront (k_step_pair): Hubble outflow of 200 km/s
+    (upwind fluxes, D:∇u work), dust, a cone, an odd N_r and segment lengths that end a row
+    with a half-filled wavefront; every step's M_F and f_esc, the J energy and the escaping
+    spectrum against the FP64 step."""
+    bcz = L.OUTFLOW if bc == "outflow" else L.REFLECT
+    cloud = developed_state(small_cloud(n_r=19, n_z=13, n_nu=n_nu, tau0=1e7, tau_abs=1.0, theta_cone=np.pi / 4,
+                                        umax=200.0, bc_z_bot=bcz, lut=lut))
+    cloud.lya.hll = hll
+    ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+    h64 = M.run_steps(ctx, 25, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=bcz)
+    b = M1LyaBatch([cloud])
+    b.set_step(0, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=bcz)
+    b.set_tiles(*tiles)
+    b.set_profiling(1)
+    hb = b.run_steps(25)[:, 0, :]
+    assert "k_step_pair" in b.kernel_stats() and "k_step" not in b.kernel_stats()
+    _check_history(hb, h64, f"pair hll={hll} bc={bc} n_nu={n_nu} tiles={tiles}")
+    ee = _energy_err(cloud, b.download(0, "J"), ctx.download("J"))
+    es = _spectrum_err(cloud, b, ctx)
+    print(f"pair mode n_nu={n_nu} hll={hll} bc={bc} tiles={tiles}: J energy {ee:.2e}, spectrum {es:.2e}")
+    assert ee <= FP32_TOL and es <= FP32_TOL
+    ctx.close()
+    b.close()
+
+
 def _spectrum_err(cloud, b, ctx, idx=0):
     """Max relative error of the escaping spectrum (the per-bin terms of f_esc) over the bins
     above 1e-3 of its peak."""
@@ -344,8 +372,10 @@ def test_wide_and_narrow_clouds_in_one_batch(lut):
 
 
 def test_profiling_modes(lut):
-    """Profiling level 2 brackets only k_step (two events per step); level 1 every launch."""
-    b = M1LyaBatch(_mixed_clouds(lut)[:1])
+    """Profiling level 2 brackets only the step kernels (two events per step); level 1 every
+    launch.  A batch holding a cloud of N_ν <= 32 and a wider one launches both k_step_pair and
+    k_step."""
+    b = M1LyaBatch(_mixed_clouds(lut)[1:2])
     b.set_profiling(2)
     b.run_steps(3, history=False)
     st = b.kernel_stats()
@@ -356,4 +386,10 @@ def test_profiling_modes(lut):
     st = b.kernel_stats()
     assert {"k_step", "k_reduce", "k_final"} <= set(st) and all(v[0] == 2 for k, v in st.items() if k in ("k_step", "k_reduce", "k_final"))
     b.set_profiling(0)
+    b.close()
+    b = M1LyaBatch(_mixed_clouds(lut)[:2])
+    b.set_profiling(2)
+    b.run_steps(2, history=False)
+    st = b.kernel_stats()
+    assert set(st) == {"k_step", "k_step_pair"} and all(v[0] == 2 for v in st.values())
     b.close()
