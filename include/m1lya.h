@@ -412,7 +412,8 @@ int  m1lya_batch_autotune(m1lya_batch *b, int trial_steps, int *seg_len, int *wa
 int  m1lya_batch_set_graph(m1lya_batch *b, int on);
 int  m1lya_batch_synchronize(m1lya_batch *b);
 /* on = 1: every launch of a step is bracketed by timing events (m1lya_batch_kernel_stats);
- * on = 2: only k_step, the step's dominant kernel (two events per step instead of six);
+ * on = 2: only the step kernels k_step and k_step_pair (clouds of N_ν <= 32), the step's
+ * dominant work (two events per launch instead of around every launch);
  * 0: none.  Graph replay is suspended while profiling. */
 int  m1lya_batch_set_profiling(m1lya_batch *b, int on);
 int  m1lya_batch_kernel_stats_count(m1lya_batch *b);
