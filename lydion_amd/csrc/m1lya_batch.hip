@@ -245,6 +245,18 @@ __device__ __forceinline__ double exp_clamped(double x)
     return __builtin_ldexp(p, (int)n);
 }
 
+// exp(x) for |x| <= kPmax to about 1e-7 relative: the power of two 2^n in FP64 and the fraction
+// 2^r, |r| <= 1/2, from the hardware's FP32 exp2 (v_exp_f32, ~1 ulp).  The frequency factors
+// are formed in conservative interface form, so an error in the exponentials moves the
+// equilibrium ratio of neighbouring bins by that error and does not break photon conservation.
+__device__ __forceinline__ double exp_fast(double x)
+{
+    constexpr double kLog2e = 1.44269504088896338700e+00;
+    const double y = x * kLog2e;
+    const double n = __builtin_rint(y);
+    return __builtin_ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
+}
+
 // OneMinusϕ_L_and_χ_L / OneMinusϕ_R_and_χ_R (src/FreqFactors.jl:54-133) in FP64
 __device__ __forceinline__ void omphi_L64(double P, double& o, double& chi)
 {
@@ -833,7 +845,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ <= 2 ? 4 : 2) void k_step(const 
                     oL = 1.0 - P + 0.5 * (P * P);
                     xL = 1.0 - 0.5 * P;
                 } else {
-                    oL = exp_clamped(-P);
+                    oL = exp_fast(-P);
                     xL = (1.0 - oL) * drcp0(P);
                 }
                 if (fabs(Pp) < kPmin) {
@@ -842,7 +854,7 @@ __global__ __launch_bounds__(64 * kMaxWaves, NQ <= 2 ? 4 : 2) void k_step(const 
                 } else {
                     // exp(P') = exp(P)·exp(P' - P); P' - P is the rounding of the bin centre
                     const double d = Pp - P;
-                    oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp_clamped(Pp);
+                    oRp = (fabs(d) < 1e-4) ? drcp1(oL) * (1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6.0)))) : exp_fast(Pp);
                     xRp = (1.0 - oRp) * drcp0(Pp);
                 }
                 Aq[q] = xL * qR;
