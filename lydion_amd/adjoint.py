@@ -14,8 +14,9 @@ density n_H, the per-cell temperature T and the source luminosity L:
     α = n_H σ0(T),  α_F = α Hvoigt(T),  ΔνD ∝ T^½,  σ0 ∝ T^-½,  js ∝ L,
 
 with the temperature derivative of the Voigt tables (Hvoigt and the core
-emission profile ϕ_ν) taken by central differences of the setup routine
-(update_hvoigt / update_phi) at a relative step of 1e-6.
+emission profile ϕ_ν) taken exactly for the tabulated profile
+(update_hvoigt_dT / update_phi_dT: the piecewise-linear tables differentiated
+segment by segment, then the quotient rule of the core normalisation).
 """
 from __future__ import annotations
 
@@ -26,7 +27,8 @@ import numpy as np
 
 from . import constants as K
 from ._lib import OUTFLOW, REFLECT, ptr
-from .lya_arrays import VoigtLUT, compute_a, compute_dnuD, compute_sigma0, update_hvoigt, update_phi
+from .lya_arrays import (VoigtLUT, compute_a, compute_dnuD, compute_sigma0, update_hvoigt, update_hvoigt_dT,
+                         update_phi, update_phi_dT)
 
 CELL_FIELDS = ("alpha", "alpha_c", "js", "dnuD", "T")
 BIN_FIELDS = ("alpha_F", "Hvoigt", "phi")
@@ -104,8 +106,16 @@ def _dust_follows_density(cloud) -> bool:
     return "kappa_abs" in cloud.params
 
 
-def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, rel_step_T: float = 1e-6,
-                       with_T: bool = True) -> dict:
+def setup_tables_dT(cloud, T: np.ndarray, lut: VoigtLUT | None = None):
+    """d Hvoigt / dT and d ϕ_ν / dT of setup_tables at temperatures T (N_Z, N_r)."""
+    L = cloud.lya
+    lut = lut or VoigtLUT()
+    Hv = update_hvoigt(lut, L["dnu_rel"], L["dnuC_rel"], compute_a(T), 1.0 / compute_dnuD(T))
+    dHv = update_hvoigt_dT(lut, L["dnu_rel"], L["dnuC_rel"], T)
+    return dHv, update_phi_dT(L["dnu"], Hv, dHv, *L.kcore)
+
+
+def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, with_T: bool = True) -> dict:
     """d Obj / d n_H (per cell), d Obj / d T (per cell; skipped when with_T is False) and
     d Obj / d L (scalar)."""
     L = cloud.lya
@@ -124,12 +134,8 @@ def physical_gradients(cloud, res: AdjointResult, lut: VoigtLUT | None = None, r
     g_L = float(np.sum(gc["js"] * L["js"])) / L_lya      # js ∝ L
     if not with_T:
         return {"n_H": g_nH, "T": None, "L": g_L}
-    # temperature: σ0 ∝ T^-1/2, ΔνD ∝ T^1/2, Voigt tables by central differences of the setup
-    h = rel_step_T * T
-    _, Hp, php = setup_tables(cloud, T + h, lut)
-    _, Hm, phm = setup_tables(cloud, T - h, lut)
-    dHv = (Hp - Hm) / (2.0 * h[..., None])
-    dphi = (php - phm) / (2.0 * h[..., None])
+    # temperature: σ0 ∝ T^-1/2, ΔνD ∝ T^1/2, Voigt tables differentiated exactly
+    dHv, dphi = setup_tables_dT(cloud, T, lut)
     dalpha = nH * (-0.5 * sig / T)
     ddnuD = 0.5 * L["dnuD"] / T
     dalphaF = dalpha[..., None] * Hv + alpha[..., None] * dHv

@@ -138,6 +138,15 @@ def _lut_eval(y: np.ndarray, nx: int, xmin: float, inv_dx: float, x: np.ndarray)
     return d * y[i] + (1.0 - d) * y[i + 1]
 
 
+def _lut_slope(y: np.ndarray, nx: int, xmin: float, inv_dx: float, x: np.ndarray) -> np.ndarray:
+    """d/dx of _lut_eval: the slope of the table segment x falls in (the segment
+    _lut_eval interpolates on, including its clamp at the top end)."""
+    xt = (x - xmin) * inv_dx
+    i = np.floor(xt).astype(np.int64)
+    i = np.where(i + 2 > nx, nx - 2, i)
+    return (y[i + 1] - y[i]) * inv_dx
+
+
 class VoigtLUT:
     def __init__(self, luts: np.ndarray | None = None, xmax: float = XMAX_VOIGT):
         self.t = voigt_luts(xmax) if luts is None else luts
@@ -159,6 +168,47 @@ class VoigtLUT:
         else:
             out = np.zeros_like(x)
         return np.where(inside, v, out)
+
+    def _dfn(self, which: int, x: np.ndarray) -> np.ndarray:
+        """d/dx of _fn: the table slope inside |x| < xmax, the derivative of the
+        asymptotic form outside it."""
+        inside = np.abs(x) < self.xmax
+        xs = np.where(inside, x, 0.0)
+        v = _lut_slope(self.t[which], self.nx, self.xmin, self.inv_dx, xs)
+        if which == 1:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ix2 = 1.0 / (x * x)
+            out = -0.5 * ix2 - 0.75 * (ix2 * ix2)
+        else:
+            out = np.zeros_like(x)
+        return np.where(inside, v, out)
+
+    def H_scale(self, x, a):
+        """(x d/dx + a d/da) H(x, a) of the tabulated H: the derivative along x, a ∝ s
+        at s = 1."""
+        f1 = self._fn(1, x)
+        dx = self._dfn(2, x) + K.inv_Sqrt_pi_over_2 * a * (2.0 * f1 + 2.0 * x * self._dfn(1, x)) \
+            + (a * a) * self._dfn(4, x)
+        da = K.inv_Sqrt_pi_over_2 * (2.0 * x * f1 - 1.0) + 2.0 * a * self._fn(4, x)
+        return x * dx + a * da
+
+    def int_H_scale(self, x, a):
+        """(x d/dx + a d/da) of the tabulated ∫_0^x H dx."""
+        dx = K.Sqrt_pi_over_2 * self._dfn(0, x) - K.inv_Sqrt_pi_over_2 * a * self._dfn(1, x) \
+            + (a * a) * self._dfn(3, x)
+        da = -K.inv_Sqrt_pi_over_2 * self._fn(1, x) + 2.0 * a * self._fn(3, x)
+        return x * dx + a * da
+
+    def htophat_scale(self, x, xmax, xmin, a):
+        """(x d/dx + xmax d/dxmax + xmin d/dxmin + a d/da) Htophat: the derivative of
+        Htophat when x, xmax, xmin and a all scale together, at scale 1.  Over a bin,
+        Htophat = (I(xmax) - I(xmin)) / (xmax - xmin) and the bin width scales too, so
+        the operator gives (DI(xmax) - DI(xmin)) / (xmax - xmin) - Htophat."""
+        dx = xmax - xmin
+        with np.errstate(invalid="ignore", divide="ignore"):
+            avg = (self.int_H(xmax, a) - self.int_H(xmin, a)) / dx
+            d_avg = (self.int_H_scale(xmax, a) - self.int_H_scale(xmin, a)) / dx - avg
+        return np.where(dx > 1e-5, d_avg, self.H_scale(x, a))
 
     def H(self, x, a):
         """H(x,a), Smith+2018 eq. 12 (src/ConstantsAndFunctions.jl:227-239)."""
@@ -191,6 +241,39 @@ def update_hvoigt(lut: VoigtLUT, dnu_rel: np.ndarray, dnuC_rel: np.ndarray, a: n
         xmn = dnu_rel[:-1] * iu
         cols[u] = lut.htophat(x, xmx, xmn, au)
     return cols[inv.reshape(-1)].reshape(a.shape + (len(dnuC_rel),))
+
+
+def update_hvoigt_dT(lut: VoigtLUT, dnu_rel: np.ndarray, dnuC_rel: np.ndarray, T: np.ndarray) -> np.ndarray:
+    """d Hvoigt / dT of update_hvoigt at the temperatures T (N_Z, N_r), exact for the
+    tabulated profile (piecewise-linear tables, differentiated segment by segment).
+    x = Δν/ΔνD and a both go as b(T)^-1 ∝ T^-1/2, so d/dT = -(x ∂x + a ∂a) / (2T)."""
+    T = np.asarray(T, dtype=np.float64)
+    a = compute_a(T)
+    iu_all = 1.0 / compute_dnuD(T)
+    keys = np.stack([a.reshape(-1), iu_all.reshape(-1), T.reshape(-1)], axis=1)
+    uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+    cols = np.empty((len(uniq), len(dnuC_rel)))
+    for u, (au, iu, Tu) in enumerate(uniq):
+        D = lut.htophat_scale(dnuC_rel * iu, dnu_rel[1:] * iu, dnu_rel[:-1] * iu, au)
+        cols[u] = -D / (2.0 * Tu)
+    return cols[inv.reshape(-1)].reshape(T.shape + (len(dnuC_rel),))
+
+
+def update_phi_dT(dnu: np.ndarray, Hvoigt: np.ndarray, dHvoigt: np.ndarray, kmin: int, kmax: int) -> np.ndarray:
+    """d ϕ_ν / dT of update_phi given Hvoigt and dHvoigt/dT: with s = Σ_core Δν_k H_k,
+    dϕ_k = (dH_k - ϕ_k ds) / s on the core bins, 0 elsewhere."""
+    flat = Hvoigt.reshape(-1, Hvoigt.shape[-1])
+    dflat = dHvoigt.reshape(-1, Hvoigt.shape[-1])
+    s = np.zeros(flat.shape[0])
+    ds = np.zeros(flat.shape[0])
+    for k in range(kmin, kmax + 1):
+        s = s + dnu[k] * flat[:, k]
+        ds = ds + dnu[k] * dflat[:, k]
+    inv = 1.0 / s
+    out = np.zeros_like(dflat)
+    phi = flat[:, kmin:kmax + 1] * inv[:, None]
+    out[:, kmin:kmax + 1] = (dflat[:, kmin:kmax + 1] - phi * ds[:, None]) * inv[:, None]
+    return out.reshape(Hvoigt.shape)
 
 
 def update_phi(dnu: np.ndarray, Hvoigt: np.ndarray, kmin: int, kmax: int) -> np.ndarray:
