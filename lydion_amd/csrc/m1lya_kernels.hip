@@ -290,35 +290,43 @@ __device__ int gtsv_col(int n, double* __restrict__ dl, double* __restrict__ d, 
 {
     double d_i = d[0], b_i = b[0];
     double du_i = (n > 1) ? du[0] : 0.0;
+    // The inputs of row i+1 are read one row ahead of their use, so the LDS latency
+    // overlaps the previous row's division instead of lengthening the chain.  No row
+    // step writes what a later read takes (it writes dl, d, du, b at row i only).
+    double dl_c = 0.0, d_c = 0.0, b_c = 0.0, du_c = 0.0;
+    if (n > 1) { dl_c = dl[0]; d_c = d[1]; b_c = b[1]; du_c = du[min(1, n - 2)]; }
+    int info = 0;
     for (int i = 0; i < n - 1; ++i) {
         const bool last = (i == n - 2);
-        const double dl_i = dl[i];
-        double d_n = d[i + 1], b_n = b[i + 1];
-        double du_n = last ? 0.0 : du[i + 1];
-        if (fabs(d_i) >= fabs(dl_i)) {
-            if (d_i == 0.0) return i + 1;
-            double fact = dl_i / d_i;
-            d_n = d_n - fact * du_i;
-            b_n = b_n - fact * b_i;
-            if (!last) dl[i] = 0.0;
-        } else {
-            double fact = d_i / dl_i;
-            double new_d_i = dl_i;
-            double temp = d_n;
-            d_n = du_i - fact * temp;
-            if (!last) {
-                dl[i] = du_n;             // fill-in of the second super-diagonal
-                du_n = -fact * du_n;
-            }
-            du_i = temp;
-            double tb = b_i;
-            b_i = b_n;
-            b_n = tb - fact * b_n;
-            d_i = new_d_i;
+        const double dl_i = dl_c, d_n = d_c, b_n = b_c, du_n = last ? 0.0 : du_c;
+        {
+            // unconditional reads, clamped to the column (what a clamped read returns is
+            // never used), so that nothing waits for them before the next row step
+            const int r1 = min(i + 1, n - 2), r2 = min(i + 2, n - 1), r3 = max(min(i + 2, n - 2), 0);
+            dl_c = dl[r1]; d_c = d[r2]; b_c = b[r2]; du_c = du[r3];
         }
-        d[i] = d_i; du[i] = du_i; b[i] = b_i;
-        d_i = d_n; b_i = b_n; du_i = du_n;
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the row step
+        // Both branches of the row step as selects around one division: without an
+        // interchange fact = dl_i/d_i, row i stays and row i+1 loses fact·row i; with
+        // one (|d_i| < |dl_i|) fact = d_i/dl_i, rows i and i+1 swap places, row i+1
+        // fills in dl[i] (the second super-diagonal) and becomes old row i - fact·row i+1.
+        // Each value is formed by the same operations as in LAPACK's two branches.
+        const bool swap = !(fabs(d_i) >= fabs(dl_i));
+        if (!swap && d_i == 0.0 && info == 0) info = i + 1;
+        const double fact = (swap ? d_i : dl_i) / (swap ? dl_i : d_i);
+        const double st_d = swap ? dl_i : d_i, st_du = swap ? d_n : du_i, st_b = swap ? b_n : b_i;
+        const double nd = (swap ? du_i : d_n) - fact * (swap ? d_n : du_i);
+        const double nb = (swap ? b_i : b_n) - fact * (swap ? b_n : b_i);
+        if (!last) {
+            dl[i] = swap ? du_n : 0.0;
+            du_i = swap ? -fact * du_n : du_n;
+        } else {
+            du_i = du_n;
+        }
+        d[i] = st_d; du[i] = st_du; b[i] = st_b;
+        d_i = nd; b_i = nb;
     }
+    if (info != 0) return info;
     if (d_i == 0.0) return n;
     d[n - 1] = d_i;
     double x1 = b_i / d_i;

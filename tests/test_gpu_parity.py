@@ -95,6 +95,44 @@ def test_frequency_update_bit_exact(lut, umax):
     _assert_state_equal(ctx, o, STATE)
 
 
+def _interchanges(dl, d, du):
+    """Row interchanges LAPACK dgtsv makes on one tridiagonal system (its pivot rule:
+    swap rows i and i+1 when |d_i| < |dl_i| after the elimination of row i-1)."""
+    d = d.copy()
+    du = np.append(du.copy(), 0.0)
+    swaps = 0
+    for i in range(len(d) - 1):
+        if abs(d[i]) >= abs(dl[i]):
+            d[i + 1] -= dl[i] / d[i] * du[i]
+        else:
+            swaps += 1
+            f = d[i] / dl[i]
+            d[i + 1], du[i + 1] = du[i] - f * d[i + 1], -f * du[i + 1]
+    return swaps
+
+
+def test_frequency_solve_with_row_interchanges_bit_exact(lut):
+    """Stored frequency factors that are no M-matrix, so dgtsv interchanges rows: the
+    solve's interchange path, which physical factors never take, against the oracle."""
+    cloud = randomize(small_cloud(lut=lut, n_nu=64, tau0=1e7), seed=21)
+    ctx, o = _pair(cloud)
+    cdt = cloud.c_tilde * cloud.dt
+    dw = 1.0 / cdt
+    shape = cloud.lya["J"].shape
+    rng = np.random.default_rng(8)
+    F = {"Fkp": dw * rng.uniform(-2.0, 2.0, shape), "Fk": dw * rng.uniform(-1.5, 0.5, shape),
+         "Fkm": dw * rng.uniform(-2.0, 2.0, shape)}
+    for k, v in F.items():
+        ctx.upload(k, v)
+        o.set(k, v)
+    # rows 1..N_ν-2 of one cell's system, as the solve forms them
+    Fkp, Fk, Fkm = (F[k][0, 0, 1:-1] for k in ("Fkp", "Fk", "Fkm"))
+    assert _interchanges(Fkm[1:], -(Fk + dw), Fkp[:-1]) > 5
+    o.frequency_update_all_cells(cdt, update_ff=False)
+    M.frequency_update_j_lya_all_cells(cdt, ctx, update_freq_factors=False)
+    _assert_state_equal(ctx, o, STATE)
+
+
 def test_reductions_match(lut):
     cloud = randomize(small_cloud(lut=lut, n_nu=32), seed=14)
     ctx, o = _pair(cloud)
