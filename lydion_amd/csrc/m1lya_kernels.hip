@@ -470,6 +470,9 @@ __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
                 const double durdr = FREQ ? a.dur_dr[c] : 0.0, durdz = FREQ ? a.dur_dz[c] : 0.0;
                 const double duzdr = FREQ ? a.duz_dr[c] : 0.0, duzdz = FREQ ? a.duz_dz[c] : 0.0;
                 const double urr = FREQ ? a.ur_over_r[c] : 0.0;
+                // With no velocity gradient D:∇u is +0 whatever D is (D finite; its rr, zz and
+                // ϕϕ entries are >= 0), so the cell's bins skip the closure
+                const bool still = durdr == 0.0 && durdz == 0.0 && duzdr == 0.0 && duzdz == 0.0 && urr == 0.0;
 #pragma unroll
                 for (int kk = 0; kk < KPL; ++kk) {
                     const int k = lane + 64 * kk;
@@ -487,9 +490,12 @@ __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
                         double two_k = 2.0 * a.inv_nuC[k];
                         double two_kp = 2.0 * a.inv_nuC[k + 1];
                         double two_km = 2.0 * a.inv_nuC[k - 1];
-                        Edd D = D_edd(Hr[w][kk], Hz[w][kk], Jcur[kk]);
-                        double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
-                        double VEL = Ddu * kInvC;
+                        double VEL = 0.0;
+                        if (!still) {
+                            Edd D = D_edd(Hr[w][kk], Hz[w][kk], Jcur[kk]);
+                            double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
+                            VEL = Ddu * kInvC;
+                        }
                         double Kkp = PREFACT * a.Hv[idx + 1], Kk = PREFACT * a.Hv[idx];
                         double Kkm = PREFACT * a.Hv[idx - 1];
                         double Ckp = VEL * nuCkp, Ck = VEL * nuCk, Ckm = VEL * nuCkm;
