@@ -538,10 +538,17 @@ __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
         if (SOLVE) {
             __syncthreads();
             const int c = grp * WB + lane;
-            if (wave == 0 && lane < WB && c < ncells && n > 0) {
-                int info = gtsv_col(n, sDL + (size_t)lane * S, sD + (size_t)lane * S, sDU + (size_t)lane * S,
-                                    sB + (size_t)lane * S);
-                if (info != 0) atomicMax(a.err_flag, 1);
+            if (wave == 0) {
+                // The solve is one dependent chain per column; the other block on the CU keeps
+                // its SIMD busy meanwhile.  At raised priority the chain's instructions issue as
+                // soon as they are ready, and the other wavefronts fill the cycles in between.
+                __builtin_amdgcn_s_setprio(3);
+                if (lane < WB && c < ncells && n > 0) {
+                    int info = gtsv_col(n, sDL + (size_t)lane * S, sD + (size_t)lane * S, sDU + (size_t)lane * S,
+                                        sB + (size_t)lane * S);
+                    if (info != 0) atomicMax(a.err_flag, 1);
+                }
+                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
         }
