@@ -160,88 +160,15 @@ __device__ __forceinline__ void load_z(const SpecIn& s, int nn, int nr, int nz, 
     }
 }
 
-// ------------------------------------------------------------------------
-// AdvectJandH_Cylindrical!: src/AdvectJandH.jl:526-1390.
-// A block is 64 bins x BI cells along r x BJ cells along Z, one wavefront per
-// cell.  Each interface is evaluated once: a wavefront evaluates its cell's left
-// r face and bottom Z face, the block's last column and row (or the cloud's last
-// cell) also the right / top face, and the results go through LDS to the two
-// cells that share them.  Then each thread applies its cell update (:1302-1382).
-// Reads the state, writes the new state to separate buffers (all fluxes of a
-// step are computed from the same old state, as in the reference).  Bins outside
-// [kmin, kmax] are copied.  A face's value does not depend on which cell's
-// wavefront evaluates it, so the result is the one of a per-cell evaluation.
-// ------------------------------------------------------------------------
-template <int BI, int BJ>
-__global__ __launch_bounds__(64 * BI * BJ) void k_advect(AdvectArgs a)
+// Cell update of AdvectJandH_Cylindrical! (src/AdvectJandH.jl:1319-1382) from the fluxes of the
+// cell's four faces (J, H_r, H_Z components; r faces left/right, Z faces bottom/top) and the
+// K_ϕϕ of the two r faces' sides that touch the cell.  Writes the new state of bin k.
+__device__ __forceinline__ void advect_cell_update(const AdvectArgs& a, int i, int j, size_t idx, double Js,
+                                                   double Hrs, double Hzs, const double (&FrL)[3],
+                                                   const double (&FrR)[3], const double (&FzL)[3],
+                                                   const double (&FzR)[3], double kphi_im, double kphi_ip)
 {
-    // r faces of the block's rows: F_J, F_Hr, F_HZ, K_ϕϕ left, K_ϕϕ right (the inner-axis
-    // value at f = 0); Z faces of its columns: F_J, F_Hr, F_HZ
-    __shared__ double rF[BJ][BI + 1][5][64];
-    __shared__ double zF[BJ + 1][BI][3][64];
-    if (a.halt && *a.halt) return;
-    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
-    const int lane = threadIdx.x, il = threadIdx.y, jl = threadIdx.z;
-    const int k = blockIdx.x * 64 + lane;
-    const int i = blockIdx.y * BI + il;
-    const int j = blockIdx.z * BJ + jl;
-    const bool cell_ok = k < nn && i < nr && j < nz;
-    const int kmin = a.krange[0], kmax = a.krange[1];
-    const bool act = cell_ok && k >= kmin && k <= kmax;
-    const SpecIn s{a.J, a.Hr, a.Hz};
-
-    auto r_face = [&](int f, int slot) {
-        Stencil st;
-        const int cl = f - 1;
-        const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
-#pragma unroll
-        for (int m = 0; m < 4; ++m) load_r(s, nn, nr, j, k, cells[m], a.bc_r_right, st.J[m], st.Hr[m], st.Hz[m]);
-        const double aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
-        const double aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
-        const double u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
-        const FaceResult r = interface_flux(0, a.hll, st, a.g.ivr + (size_t)f * 5, aFL, aFR, u);
-        double* o = &rF[jl][slot][0][lane];
-        o[0] = r.F.J; o[64] = r.F.Hr; o[128] = r.F.Hz;
-        o[192] = r.kphi_L;
-        o[256] = (f == 0) ? kphi_inner_axis(r.J_R, r.Hz_R) : r.kphi_R;
-    };
-    auto z_face = [&](int g, int slot) {
-        Stencil st;
-        const int cl = g - 1;
-        const int cells[4] = {cl, cl + 1, cl + 2, cl - 1};
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-            load_z(s, nn, nr, nz, i, k, cells[m], a.bc_z_bot, a.bc_z_top, st.J[m], st.Hr[m], st.Hz[m]);
-        const double aFL = a.aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
-        const double aFR = a.aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
-        const double u = a.v.uz_star[(size_t)g * nr + i] * kInvC;
-        const FaceResult r = interface_flux(1, a.hll, st, a.g.ivz + (size_t)g * 5, aFL, aFR, u);
-        double* o = &zF[slot][il][0][lane];
-        o[0] = r.F.J; o[64] = r.F.Hr; o[128] = r.F.Hz;
-    };
-    if (act) {
-        r_face(i, il);
-        if (il == BI - 1 || i == nr - 1) r_face(i + 1, il + 1);
-        z_face(j, jl);
-        if (jl == BJ - 1 || j == nz - 1) z_face(j + 1, jl + 1);
-    }
-    __syncthreads();
-    if (!cell_ok) return;
-    const size_t idx = ((size_t)j * nr + i) * nn + k;
-    if (!act) {
-        a.J_out[idx] = s.J[idx];
-        a.Hr_out[idx] = s.Hr[idx];
-        a.Hz_out[idx] = s.Hz[idx];
-        return;
-    }
-    const double FrL0 = rF[jl][il][0][lane], FrL1 = rF[jl][il][1][lane], FrL2 = rF[jl][il][2][lane];
-    const double FrR0 = rF[jl][il + 1][0][lane], FrR1 = rF[jl][il + 1][1][lane], FrR2 = rF[jl][il + 1][2][lane];
-    const double FzL0 = zF[jl][il][0][lane], FzL1 = zF[jl][il][1][lane], FzL2 = zF[jl][il][2][lane];
-    const double FzR0 = zF[jl + 1][il][0][lane], FzR1 = zF[jl + 1][il][1][lane], FzR2 = zF[jl + 1][il][2][lane];
-    const double kphi_im = rF[jl][il][4][lane];
-    const double kphi_ip = rF[jl][il + 1][3][lane];
-
-    // ---- cell update: src/AdvectJandH.jl:1319-1379 ----
+    const int nr = a.g.nr;
     const int c = j * nr + i;
     const double Ar_L = a.g.A_r[(size_t)j * (nr + 1) + i], Ar_R = a.g.A_r[(size_t)j * (nr + 1) + i + 1];
     const double Az_L = a.g.A_z[(size_t)j * nr + i], Az_R = a.g.A_z[(size_t)(j + 1) * nr + i];
@@ -250,12 +177,11 @@ __global__ __launch_bounds__(64 * BI * BJ) void k_advect(AdvectArgs a)
     const double cdt_over_c = cdt * kInvC;
     const double durdr = a.v.dur_dr[c], durdz = a.v.dur_dz[c], duzdr = a.v.duz_dr[c];
     const double duzdz = a.v.duz_dz[c], urr = a.v.ur_over_r[c];
-    const double Js = s.J[idx], Hrs = s.Hr[idx], Hzs = s.Hz[idx];
     Edd D = D_edd(Hrs, Hzs, Js);
-    double Jn = Js - cdt_dV * ((Ar_R * FrR0 - Ar_L * FrL0) + (Az_R * FzR0 - Az_L * FzL0));
+    double Jn = Js - cdt_dV * ((Ar_R * FrR[0] - Ar_L * FrL[0]) + (Az_R * FzR[0] - Az_L * FzL[0]));
     Jn = dmax(Jn, kJeps);
-    double Hzn = Hzs - cdt_dV * ((Ar_R * FrR2 - Ar_L * FrL2) + (Az_R * FzR2 - Az_L * FzL2));
-    double Hrn = Hrs - cdt_dV * ((Ar_R * FrR1 - Ar_L * FrL1) + (Az_R * FzR1 - Az_L * FzL1));
+    double Hzn = Hzs - cdt_dV * ((Ar_R * FrR[2] - Ar_L * FrL[2]) + (Az_R * FzR[2] - Az_L * FzL[2]));
+    double Hrn = Hrs - cdt_dV * ((Ar_R * FrR[1] - Ar_L * FrL[1]) + (Az_R * FzR[1] - Az_L * FzL[1]));
     double Ddu = D_colon_grad_u(D, durdr, durdz, duzdr, duzdz, urr);
     double Kav = 0.5 * (kphi_im + kphi_ip);
     Hrn += cdt * Kav * a.g.inv_r_cell[i];
@@ -275,6 +201,133 @@ __global__ __launch_bounds__(64 * BI * BJ) void k_advect(AdvectArgs a)
     a.J_out[idx] = Jn;
     a.Hr_out[idx] = Hrn;
     a.Hz_out[idx] = Hzn;
+}
+
+// ------------------------------------------------------------------------
+// AdvectJandH_Cylindrical!: src/AdvectJandH.jl:526-1390, as a march along r.
+// A block is WPB rows (one wavefront each) x one segment of seg_len cells along r
+// x 64 bins (lanes).  Each wavefront keeps the r window (cells i-1 .. i+2 of its
+// row) in registers and evaluates one r face per cell; the left face of a cell
+// is the right face of the one before it.  The Z faces of the block's rows go
+// through LDS: every wavefront publishes its cell's state, wavefront w evaluates
+// face j0 + w and wavefront 0 also the block's top face.  That is 1 + 1/seg_len
+// r faces and 1 + 1/WPB Z faces per cell.  All fluxes of a step are computed from
+// the same old state (read from the input buffers, written to separate ones), as
+// in the reference; bins outside [kmin, kmax] are copied.  A face's value does not
+// depend on which wavefront evaluates it, so the result is the one of a per-cell
+// evaluation.
+// ------------------------------------------------------------------------
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_advect(AdvectArgs a, int seg_len)
+{
+    // Z stencil rows j0-2 .. j0+WPB+1 of the current column (J, H_r, H_Z, ghost-mapped) and the
+    // Z faces j0 .. j0+WPB (F_J, F_Hr, F_HZ)
+    __shared__ double zrow[WPB + 4][3][64];
+    __shared__ double zflx[WPB + 1][3][64];
+    if (a.halt && *a.halt) return;
+    const int nn = a.nn, nr = a.g.nr, nz = a.g.nz;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int k = blockIdx.x * 64 + lane;
+    const int i0 = blockIdx.y * seg_len, i1 = min(i0 + seg_len, nr);
+    const int j0 = blockIdx.z * WPB, j = j0 + w;
+    const bool row_ok = j < nz;
+    const bool kin = k < nn;
+    const int kmin = a.krange[0], kmax = a.krange[1];
+    const bool act = kin && k >= kmin && k <= kmax;
+    const SpecIn s{a.J, a.Hr, a.Hz};
+
+    // r window of row j: cells (i-1, i, i+1, i+2) in slots 0..3; before the march, cells
+    // i0-2, i0-1, i0 in slots 1..3, so that the first shift and load make (i0-2 .. i0+1)
+    double wJ[4] = {0.0, 0.0, 0.0, 0.0}, wHr[4] = {0.0, 0.0, 0.0, 0.0}, wHz[4] = {0.0, 0.0, 0.0, 0.0};
+    if (act && row_ok) {
+#pragma unroll
+        for (int m = 1; m < 4; ++m) load_r(s, nn, nr, j, k, i0 - 3 + m, a.bc_r_right, wJ[m], wHr[m], wHz[m]);
+    }
+    // left r face of cell i: its flux and the K_ϕϕ of its right side
+    double FL[3] = {0.0, 0.0, 0.0}, KL = 0.0;
+    for (int i = i0 - 1; i < i1; ++i) {
+        double FR[3] = {0.0, 0.0, 0.0}, KRl = 0.0, KRr = 0.0;
+        if (act && row_ok) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) { wJ[m] = wJ[m + 1]; wHr[m] = wHr[m + 1]; wHz[m] = wHz[m + 1]; }
+            load_r(s, nn, nr, j, k, i + 2, a.bc_r_right, wJ[3], wHr[3], wHz[3]);
+            // face f = i + 1 between cells i and i+1; stencil (i, i+1, i+2, i-1)
+            const int f = i + 1;
+            Stencil st;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int slot = (m + 1) & 3;
+                st.J[m] = wJ[slot]; st.Hr[m] = wHr[slot]; st.Hz[m] = wHz[slot];
+            }
+            const double aFL = a.aF[((size_t)j * nr + max(f - 1, 0)) * nn + k];
+            const double aFR = a.aF[((size_t)j * nr + min(f, nr - 1)) * nn + k];
+            const double u = a.v.ur_star[(size_t)j * (nr + 1) + f] * kInvC;
+            const FaceResult r = interface_flux(0, a.hll, st, a.g.ivr + (size_t)f * 5, aFL, aFR, u);
+            FR[0] = r.F.J; FR[1] = r.F.Hr; FR[2] = r.F.Hz;
+            KRl = r.kphi_L;
+            KRr = (f == 0) ? kphi_inner_axis(r.J_R, r.Hz_R) : r.kphi_R;
+        }
+        if (i < i0) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) FL[m] = FR[m];
+            KL = KRr;
+            continue;
+        }
+        // ---- Z faces of column i through LDS ----
+        if (act) {
+            if (row_ok) {
+                zrow[w + 2][0][lane] = wJ[1]; zrow[w + 2][1][lane] = wHr[1]; zrow[w + 2][2][lane] = wHz[1];
+            } else {
+                double vj, vr, vz;   // a ghost row above the grid (rows past N_Z + 1 feed no needed face)
+                load_z(s, nn, nr, nz, i, k, min(j, nz + 1), a.bc_z_bot, a.bc_z_top, vj, vr, vz);
+                zrow[w + 2][0][lane] = vj; zrow[w + 2][1][lane] = vr; zrow[w + 2][2][lane] = vz;
+            }
+            for (int h = w; h < 4; h += WPB) {
+                const int slot = (h < 2) ? h : WPB + h;
+                double vj, vr, vz;
+                load_z(s, nn, nr, nz, i, k, min(j0 - 2 + slot, nz + 1), a.bc_z_bot, a.bc_z_top, vj, vr, vz);
+                zrow[slot][0][lane] = vj; zrow[slot][1][lane] = vr; zrow[slot][2][lane] = vz;
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll 1
+            for (int it = 0; it < 2; ++it) {
+                if (it && w != 0) break;
+                const int fz = it ? WPB : w;    // the face's slot: face g = j0 + fz
+                const int g = j0 + fz;
+                if (g > nz) continue;
+                Stencil st;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int slot = fz + ((m + 1) & 3);   // rows (g-1, g, g+1, g-2)
+                    st.J[m] = zrow[slot][0][lane]; st.Hr[m] = zrow[slot][1][lane]; st.Hz[m] = zrow[slot][2][lane];
+                }
+                const double aFL = a.aF[((size_t)max(g - 1, 0) * nr + i) * nn + k];
+                const double aFR = a.aF[((size_t)min(g, nz - 1) * nr + i) * nn + k];
+                const double u = a.v.uz_star[(size_t)g * nr + i] * kInvC;
+                const FaceResult r = interface_flux(1, a.hll, st, a.g.ivz + (size_t)g * 5, aFL, aFR, u);
+                zflx[fz][0][lane] = r.F.J; zflx[fz][1][lane] = r.F.Hr; zflx[fz][2][lane] = r.F.Hz;
+            }
+        }
+        __syncthreads();
+        if (row_ok && kin) {
+            const size_t idx = ((size_t)j * nr + i) * nn + k;
+            if (!act) {
+                a.J_out[idx] = s.J[idx];
+                a.Hr_out[idx] = s.Hr[idx];
+                a.Hz_out[idx] = s.Hz[idx];
+            } else {
+                const double FzL[3] = {zflx[w][0][lane], zflx[w][1][lane], zflx[w][2][lane]};
+                const double FzR[3] = {zflx[w + 1][0][lane], zflx[w + 1][1][lane], zflx[w + 1][2][lane]};
+                advect_cell_update(a, i, j, idx, wJ[1], wHr[1], wHz[1], FL, FR, FzL, FzR, KL, KRl);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) FL[m] = FR[m];
+        KL = KRr;
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -844,10 +897,11 @@ void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, in
 
 void launch_advect(hipStream_t s, const AdvectArgs& a)
 {
-    // 2 x 4 cells: 2.75 face evaluations per cell instead of 4, 46 KB of LDS, two blocks per CU
-    constexpr int BI = 2, BJ = 4;
-    dim3 grid((a.nn + 63) / 64, (a.g.nr + BI - 1) / BI, (a.g.nz + BJ - 1) / BJ);
-    hipLaunchKernelGGL((k_advect<BI, BJ>), grid, dim3(64, BI, BJ), 0, s, a);
+    // rows of 8 wavefronts marching over segments of 16 cells: 1.0625 r faces and 1.125 Z faces
+    // per cell, 32 KB of LDS per block
+    constexpr int WPB = 8, SEG = 16;
+    dim3 grid((a.nn + 63) / 64, (a.g.nr + SEG - 1) / SEG, (a.g.nz + WPB - 1) / WPB);
+    hipLaunchKernelGGL((k_advect<WPB>), grid, dim3(64 * WPB), 0, s, a, SEG);
 }
 
 template <int W, int KPL, int NW>
