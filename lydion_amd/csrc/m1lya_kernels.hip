@@ -897,11 +897,17 @@ void launch_active_final(hipStream_t s, const double* red, int nn, int kcmin, in
 
 void launch_advect(hipStream_t s, const AdvectArgs& a)
 {
-    // rows of 8 wavefronts marching over segments of 16 cells: 1.0625 r faces and 1.125 Z faces
-    // per cell, 32 KB of LDS per block
-    constexpr int WPB = 8, SEG = 16;
-    dim3 grid((a.nn + 63) / 64, (a.g.nr + SEG - 1) / SEG, (a.g.nz + WPB - 1) / WPB);
-    hipLaunchKernelGGL((k_advect<WPB>), grid, dim3(64 * WPB), 0, s, a, SEG);
+    // Rows of 8 wavefronts marching over segments of 16 cells: 1.0625 r faces and 1.125 Z faces
+    // per cell, 32 KB of LDS per block.  Segments are halved while the grid holds fewer blocks
+    // than MI355X keeps resident (2 per CU at 123 VGPRs, 512 in all): on the 128x128x128
+    // expanding cloud 8-cell segments take 0.110 ms against 0.120 ms for 16 (256 blocks), on
+    // the 256x256x128 cone cloud 16 and 8 are within 1%.
+    constexpr int WPB = 8, kResident = 512;
+    const int chunks = (a.nn + 63) / 64, rows = (a.g.nz + WPB - 1) / WPB;
+    int seg = 16;
+    while (seg > 4 && chunks * ((a.g.nr + seg - 1) / seg) * rows < kResident) seg /= 2;
+    dim3 grid(chunks, (a.g.nr + seg - 1) / seg, rows);
+    hipLaunchKernelGGL((k_advect<WPB>), grid, dim3(64 * WPB), 0, s, a, seg);
 }
 
 template <int W, int KPL, int NW>
