@@ -159,7 +159,7 @@ __device__ m1a::FaceT<T> face_at(const AdjAdvArgs& a, int dir, int f, int i, int
 // Cell update of AdvectJandH_Cylindrical! (src/AdvectJandH.jl:1302-1382) in reverse.
 // ---------------------------------------------------------------------------
 // Block of 64 bins x BI x BJ cells, a wavefront per cell; the forward values of each interface
-// are evaluated once (as k_advect does) and reach the two cells that share it through LDS.
+// are evaluated once per block and reach the two cells that share it through LDS.
 template <int BI, int BJ>
 __global__ __launch_bounds__(64 * BI * BJ) void k_adj_cell(AdjAdvArgs a)
 {
@@ -874,7 +874,7 @@ void launch_adj_column(hipStream_t s, const AdjColArgs& a)
 void launch_adj_advect(hipStream_t s, const AdjAdvArgs& a)
 {
     dim3 grid((a.nn + 63) / 64, (a.g.nr + 3) / 4, a.g.nz);
-    constexpr int BI = 2, BJ = 4;   // as k_advect: 2.75 interface evaluations per cell
+    constexpr int BI = 2, BJ = 4;   // 2.75 interface evaluations per cell
     hipLaunchKernelGGL((k_adj_cell<BI, BJ>), dim3((a.nn + 63) / 64, (a.g.nr + BI - 1) / BI, (a.g.nz + BJ - 1) / BJ),
                        dim3(64, BI, BJ), 0, s, a);
     if (a.mode == 1) {
