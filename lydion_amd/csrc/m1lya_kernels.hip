@@ -409,7 +409,12 @@ __device__ int gtsv_col(int n, double* __restrict__ dl, double* __restrict__ d, 
 // LDS per block: 4 arrays x (N_ν-2) rows x W·NW columns (column index fastest, odd row
 // stride, so the lane-per-column sweep is bank-conflict free).
 // ------------------------------------------------------------------------
-template <int W, int KPL, int NW>
+// kFused: the instantiation for the fused step's phases (emission, frequency factors, solve,
+// H correction and reductions, kFusedFlags), with the phase tests resolved at compile time;
+// only COL_CONV is read from the arguments.
+constexpr int kFusedFlags = COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED;
+
+template <int W, int KPL, int NW, bool kFused>
 __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
 {
     extern __shared__ double lds[];
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int WB = W * NW;   // columns of the block
-    const int flags = a.flags;
+    const int flags = kFused ? (kFusedFlags | (a.flags & COL_CONV)) : a.flags;
     const bool EMIT = flags & COL_EMIT, FREQ = flags & COL_FREQ, STOREF = flags & COL_STORE_F;
     const bool SOLVE = flags & COL_SOLVE, HCORR = flags & COL_HCORR, FUSED = flags & COL_FUSED;
     const bool CONV = flags & COL_CONV;
@@ -916,7 +921,10 @@ static void launch_column_t(hipStream_t s, const ColumnArgs& a, int nblocks)
     const size_t rows = 4 * (size_t)col_stride(a.nn) * W * NW;
     const size_t sums = (size_t)NW * (2 * a.nn + kStepSums);
     const size_t lds = std::max(rows, sums) * sizeof(double);
-    hipLaunchKernelGGL((k_column<W, KPL, NW>), dim3(nblocks), dim3(64 * NW), lds, s, a);
+    if ((a.flags & ~COL_CONV) == kFusedFlags)
+        hipLaunchKernelGGL((k_column<W, KPL, NW, true>), dim3(nblocks), dim3(64 * NW), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_column<W, KPL, NW, false>), dim3(nblocks), dim3(64 * NW), lds, s, a);
 }
 
 // Up to 256 bins (four per lane) four cell columns share a wavefront; beyond, up to 576 bins
