@@ -82,10 +82,16 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const double* __restrict__ 
 }
 
 // Active-bin decision (src/UpdateLyaJandH.jl:73-96) from reduced e1/e10.
-__device__ void active_range(const double* red, int nn, int kcore_min, int kcore_max, int* krange)
+// E_tot = Σ_k e1[k], summed in bin order
+__device__ double active_energy(const double* red, int nn)
 {
     double Etot = 0.0;
     for (int k = 0; k < nn; ++k) Etot += red[k];
+    return Etot;
+}
+
+__device__ void active_range(const double* red, int nn, int kcore_min, int kcore_max, int* krange, double Etot)
+{
     int kmn = nn - 1, kmx = 0;
     for (int k = 0; k < nn; ++k) {
         if (red[nn + k] > kEpsActive * Etot) {
@@ -99,10 +105,22 @@ __device__ void active_range(const double* red, int nn, int kcore_min, int kcore
     krange[1] = kmx;
 }
 
+// The reduced row (2 N_ν bin sums and the step sums) copied into LDS by the whole block, so that
+// the serial tail that follows reads it at LDS latency instead of one global load per bin.
+constexpr int kRedMax = 2 * kMaxBinsFP64 + kStepSums;
+
+__device__ __forceinline__ void stage_red(double* sred, const double* red, int n)
+{
+    for (int t = threadIdx.x; t < n; t += blockDim.x) sred[t] = red[t];
+    __syncthreads();
+}
+
 __global__ void k_active_final(const double* __restrict__ red, int nn, int kcore_min, int kcore_max,
                                int* __restrict__ krange)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) active_range(red, nn, kcore_min, kcore_max, krange);
+    __shared__ double sred[kRedMax];
+    stage_red(sred, red, 2 * nn);
+    if (threadIdx.x == 0 && blockIdx.x == 0) active_range(sred, nn, kcore_min, kcore_max, krange, active_energy(sred, nn));
 }
 
 // ------------------------------------------------------------------------
@@ -700,13 +718,16 @@ __global__ __launch_bounds__(64 * NW) void k_column(ColumnArgs a)
 // Step scalars from the reduced partials of the fused column kernel:
 // Update_f_Lyα_MF_L_Lyα! normalisation (src/ConstantsAndFunctions.jl:663-673),
 // Compute_f_esc_Lyα_Cylindrical (:969-993) and FindActiveBins for the next step.
-__global__ void k_step_final(const double* __restrict__ red, int nn, int kcore_min, int kcore_max,
+__global__ void k_step_final(const double* __restrict__ red_g, int nn, int kcore_min, int kcore_max,
                              int bc_z_bot, int* __restrict__ krange, double* __restrict__ scalars,
                              double* __restrict__ history, int* __restrict__ err_flag, StepCtl* __restrict__ ctl,
                              double tol, int conv)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sred[kRedMax];
     if (ctl && ctl->halt) return;
+    stage_red(sred, red_g, 2 * nn + kStepSums);
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double* red = sred;
     const int kmin_used = krange[0], kmax_used = krange[1];
     double MF = red[2 * nn + SUM_MF], L = red[2 * nn + SUM_L], net = red[2 * nn + SUM_NET];
     double out = red[2 * nn + SUM_OUT];
@@ -725,18 +746,15 @@ __global__ void k_step_final(const double* __restrict__ red, int nn, int kcore_m
     }
     // ΔJ/J = Σ Δν ΔV |J_new - J_start| / E_tot, E_tot = Σ Δν ΔV max(J_new - Jeps, 0) (the
     // energy FindActiveBins measures, src/UpdateLyaJandH.jl:60-66)
+    const double Etot = active_energy(red, nn);
     double rel = 0.0;
-    if (conv) {
-        double Etot = 0.0;
-        for (int k = 0; k < nn; ++k) Etot += red[k];
-        rel = (Etot > 0.0) ? red[2 * nn + SUM_CONV] / Etot : 0.0;
-    }
+    if (conv) rel = (Etot > 0.0) ? red[2 * nn + SUM_CONV] / Etot : 0.0;
     if (red[2 * nn + SUM_BAD] != 0.0) atomicOr(err_flag, (int)ERR_NONFINITE);
     double v[8] = {MF, net, out, Ltot, fesc, (double)kmin_used, (double)kmax_used, rel};
     for (int m = 0; m < 8; ++m) scalars[m] = v[m];
     if (history)
         for (int m = 0; m < 8; ++m) history[m] = v[m];
-    active_range(red, nn, kcore_min, kcore_max, krange);
+    active_range(red, nn, kcore_min, kcore_max, krange, Etot);
     if (ctl) {
         ctl->steps += 1;
         if (tol > 0.0 && conv && rel < tol) ctl->halt = 1;
