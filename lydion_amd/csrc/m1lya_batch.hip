@@ -1200,13 +1200,22 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
     __shared__ double col[kPartStride];
     __shared__ double tree[256];
     __shared__ double vout[8];
-    __shared__ int kmn_s, kmx_s, sstep;
+    __shared__ int kmn_s, kmx_s, sstep, skr[2], ssteady, scap;
+    __shared__ double stol;
+    __shared__ double* shist;
     if (!ctrl->live) return;
     const int c = blockIdx.x;
     const Cloud& cl = clouds[c];
     const int nn = cl.nn, ncol = 2 * nn + kNScal, t = threadIdx.x;
-    // the serial tail reads the cursor: load it while the columns are summed
+    // the serial tail's inputs from global memory, loaded by a thread each while the columns
+    // are summed, so that none of them is a dependent load on the tail
     if (t == 0) sstep = ctrl->step;
+    else if (t == 1) skr[0] = krange[2 * c];
+    else if (t == 2) skr[1] = krange[2 * c + 1];
+    else if (t == 3) stol = ctrl->tol;
+    else if (t == 4) ssteady = (ctrl->tol > 0.0) ? ctrl->steady[c] : 0;   // read only with a stop tolerance
+    else if (t == 5) shist = ctrl->hist;
+    else if (t == 6) scap = ctrl->cap;
     for (int m = t; m < ncol; m += blockDim.x) {
         KSum acc{0, 0};
 #pragma unroll
@@ -1246,9 +1255,9 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
         // ΔJ/J = Σ Δν ΔV |J_new - J_start| / E_tot (E_tot: the energy FindActiveBins measures)
         const double rel = (Etot > 0.0) ? col[2 * nn + 5] / Etot : 0.0;
         vout[0] = MF; vout[1] = nt; vout[2] = ow; vout[3] = Ltot; vout[4] = fesc;
-        vout[5] = (double)krange[2 * c]; vout[6] = (double)krange[2 * c + 1]; vout[7] = rel;
+        vout[5] = (double)skr[0]; vout[6] = (double)skr[1]; vout[7] = rel;
         if (col[2 * nn + 6] != 0.0 && atomicCAS(&ctrl->bad, 0, c + 1) == 0) ctrl->bad_step = step;
-        if (ctrl->tol > 0.0 && rel < ctrl->tol && ctrl->steady[c] == 0) {
+        if (stol > 0.0 && rel < stol && ssteady == 0) {
             ctrl->steady[c] = step + 1;
             if (atomicAdd(&ctrl->nsteady, 1) == n - 1) ctrl->halt = 1;
         }
@@ -1258,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_final(const Cloud* __restrict__ clouds,
     if (!active_only && t < 8) {
         // the scalars and the history row, one value per thread
         scalars[(size_t)c * 8 + t] = vout[t];
-        if (ctrl->hist && step < ctrl->cap) ctrl->hist[((size_t)step * n + c) * 8 + t] = vout[t];
+        if (shist && step < scap) shist[((size_t)step * n + c) * 8 + t] = vout[t];
     }
     if (t != 0) return;
     krange[2 * c] = min(kmn_s, cl.kcore_min);
