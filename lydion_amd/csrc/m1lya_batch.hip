@@ -1160,23 +1160,40 @@ __global__ __launch_bounds__(256) void k_reduce(const Cloud* __restrict__ clouds
                                                 double* __restrict__ red, Ctrl* __restrict__ ctrl)
 {
     __shared__ double ss[4][64], sc[4][64];
-    const bool halted = ctrl->halt != 0;
+    // the halt flag and the cloud's extents are loaded together (one memory latency)
+    const int halt = ctrl->halt;
+    const Cloud& cl = clouds[blockIdx.y];
+    const int nn = cl.nn, blk0 = cl.blk0, blk1 = cl.blk1;
+    const bool halted = halt != 0;
     if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
         ctrl->live = !halted;
         if (!halted) ctrl->step = ctrl->next;
     }
     if (halted) return;
-    const Cloud& cl = clouds[blockIdx.y];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + lane;
-    const int ncol = 2 * cl.nn + kNScal;
+    const int ncol = 2 * nn + kNScal;
     if ((int)blockIdx.x * 64 >= ncol) return;
-    const int nrow = cl.blk1 - cl.blk0;
+    const int nrow = blk1 - blk0;
     const int per = (nrow + kSlices - 1) / kSlices;
-    const int r0 = cl.blk0 + (int)blockIdx.z * per, r1 = min(cl.blk1, r0 + per);
+    const int r0 = blk0 + (int)blockIdx.z * per, r1 = min(blk1, r0 + per);
     KSum acc{0, 0};
-    if (col < ncol)
-        for (int r = r0 + wave; r < r1; r += 4) acc.add(part[(size_t)r * kPartStride + col]);
+    if (col < ncol) {
+        // kRows rows of the wavefront's share are loaded before any is added, so their loads
+        // overlap; the compensated sum takes them in row order, as one row at a time would
+        constexpr int kRows = 8;
+        for (int r = r0 + wave; r < r1; r += 4 * kRows) {
+            double v[kRows];
+#pragma unroll
+            for (int q = 0; q < kRows; ++q) {
+                const int rq = min(r + 4 * q, r1 - 1);
+                v[q] = part[(size_t)rq * kPartStride + col];
+            }
+#pragma unroll
+            for (int q = 0; q < kRows; ++q)
+                if (r + 4 * q < r1) acc.add(v[q]);
+        }
+    }
     ss[wave][lane] = acc.s;
     sc[wave][lane] = acc.c;
     __syncthreads();
