@@ -444,9 +444,11 @@ int m1lya_find_active_bins_noalloc(const double *J, const double *dnu, const dou
 int m1lya_noalloc_last_error(char *buf, size_t len);
 
 /* ---- instrumentation ------------------------------------------------- */
-/* When on, every kernel launch is bracketed by HIP events on the context's
- * stream; m1lya_kernel_stats() reports per-kernel launch counts and total
- * device milliseconds (synchronises). */
+/* on = 1: every kernel launch is bracketed by HIP events on the context's
+ * stream; on = 2: only the column kernel of the fused step (its dominant
+ * kernel), so that the events add little to a timed run; 0: off.
+ * m1lya_kernel_stats() reports per-kernel launch counts and total device
+ * milliseconds (synchronises). */
 int m1lya_set_profiling(m1lya_ctx *ctx, int on);
 int m1lya_kernel_stats_count(m1lya_ctx *ctx);
 int m1lya_kernel_stats(m1lya_ctx *ctx, int index, char *name, size_t name_len,

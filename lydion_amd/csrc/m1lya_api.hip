@@ -89,7 +89,7 @@ struct m1lya_ctx {
     std::vector<void*> allocs;
     std::string err;
 
-    bool profiling = false;
+    int profiling = 0;   // 0 off, 1 every launch, 2 the fused step's column kernel only
     std::map<std::string, KStat> stats;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -148,9 +148,12 @@ struct Prof {
     m1lya_ctx* c;
     const char* name;
     hipEvent_t e0{}, e1{};
-    Prof(m1lya_ctx* c_, const char* n) : c(c_), name(n)
+    bool on;
+    // dominant: the launch is bracketed at profiling level 2 as well as at level 1
+    Prof(m1lya_ctx* c_, const char* n, bool dominant = false)
+        : c(c_), name(n), on(c_->profiling == 1 || (c_->profiling == 2 && dominant))
     {
-        if (c->profiling) {
+        if (on) {
             e0 = get_event(c);
             e1 = get_event(c);
             hipchk(c, hipEventRecord(e0, c->stream), "hipEventRecord");
@@ -159,7 +162,7 @@ struct Prof {
     void done()
     {
         hipchk(c, hipGetLastError(), name);
-        if (c->profiling) {
+        if (on) {
             hipchk(c, hipEventRecord(e1, c->stream), "hipEventRecord");
             c->pending.push_back(Pending{name, e0, e1});
         }
@@ -379,7 +382,7 @@ void fused_step(m1lya_ctx* c, double c_tilde, double dt, double z_center, int bc
     ColumnArgs a = column_args(c, COL_EMIT | COL_FREQ | COL_SOLVE | COL_HCORR | COL_FUSED | (run.conv ? COL_CONV : 0));
     a.c_tilde = c_tilde; a.dt = dt; a.dw = 1.0 / cdt; a.z_center = z_center; a.bc_z_bot = bc_z_bot;
     a.halt = halt;
-    Prof p(c, "column");
+    Prof p(c, "column", true);
     launch_column(c->stream, a, c->col_blocks);
     p.done();
     Prof p2(c, "reduce_cols");
@@ -1438,7 +1441,7 @@ int m1lya_set_profiling(m1lya_ctx* c, int on)
 {
     return guarded(c, [&] {
         collect_stats(c);
-        c->profiling = on != 0;
+        c->profiling = (on == 2) ? 2 : (on != 0 ? 1 : 0);
     });
 }
 

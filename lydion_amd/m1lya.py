@@ -113,7 +113,9 @@ class M1LyaContext:
         self._check(self.lib.m1lya_synchronize(self.handle))
 
     # ---- profiling ----
-    def set_profiling(self, on: bool) -> None:
+    def set_profiling(self, on) -> None:
+        """0/False off; 1/True every launch bracketed by HIP events; 2 only the fused step's
+        column kernel."""
         self._check(self.lib.m1lya_set_profiling(self.handle, int(on)))
 
     def set_adjoint_mode(self, mode: int) -> None:

@@ -133,3 +133,30 @@ def test_batch_nonfinite_state_is_reported(lut):
     with pytest.raises(M1LyaError) as ei:
         b.run_steps(3)
     assert ei.value.code == ERR_NONFINITE and "cloud 1" in str(ei.value)
+
+
+def test_fp64_profiling_levels(lut):
+    """Profiling level 1 brackets every launch (the fused step's four, and the active-bin
+    search before the first step), level 2 only the fused step's column kernel; neither
+    changes the state."""
+    cloud = small_cloud(lut=lut)
+    step_kernels = {"advect", "column", "reduce_cols", "step_final"}
+    states = []
+    for level in (0, 1, 2):
+        ctx = M.M1LyaContext(cloud.lya, cloud.geometry)
+        ctx.set_profiling(level)
+        M.run_steps(ctx, 3, cloud.c_tilde, cloud.dt, cloud.Zc, bc_z_bot=cloud.bc_z_bot, history=False)
+        st = ctx.kernel_stats()
+        if level == 0:
+            assert st == {}
+        elif level == 1:
+            assert set(st) == step_kernels | {"active_partial", "active_final"}
+            # reduce_cols also sums the active-bin search's partials
+            assert {k: st[k][0] for k in step_kernels} == {"advect": 3, "column": 3, "reduce_cols": 4, "step_final": 3}
+        else:
+            assert set(st) == {"column"} and st["column"][0] == 3
+        assert all(v[1] > 0.0 for v in st.values())
+        ctx.set_profiling(0)
+        states.append(ctx.download("J"))
+        ctx.close()
+    assert bitdiff(states[0], states[1]) == 0 and bitdiff(states[0], states[2]) == 0
